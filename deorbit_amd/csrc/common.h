@@ -1,0 +1,177 @@
+// Device-side data layout of the SPH step.
+//
+// Particles live in structure-of-arrays form in HBM and are re-sorted every
+// step into cell order (Morton order inside 4x4x4-cell bricks, bricks in
+// row-major order).  Per-particle state that must survive a step ("carried"
+// state) is double-buffered so the permutation is a pure gather; everything
+// recomputed each step lives in single buffers in the current order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "physics.h"
+
+namespace dsph {
+
+// Packed per-particle descriptor: material (8 bits) | fixed (1 bit) << 8 | segment << 16.
+__device__ __host__ __forceinline__ uint32_t info_pack(int mat, int fixed, int seg) {
+    return (uint32_t)(mat & 0xff) | ((uint32_t)(fixed & 1) << 8) | ((uint32_t)seg << 16);
+}
+__device__ __host__ __forceinline__ int info_mat(uint32_t v) { return (int)(v & 0xffu); }
+__device__ __host__ __forceinline__ int info_fixed(uint32_t v) { return (int)((v >> 8) & 1u); }
+__device__ __host__ __forceinline__ int info_seg(uint32_t v) { return (int)(v >> 16); }
+
+// Per-segment parameters (host-provided, read-only on device).
+struct SegPar {
+    double eta, h_init, alpha, beta, eps, alpha_u, cfl, eps_as, g, damp, t_end;
+    double dlo[3], dhi[3];
+    int32_t damage_on;
+    int32_t per[3];
+};
+
+// Per-segment sort grid and oracle-emulation grid (written on device).
+struct SegGrid {
+    double lo[3];
+    double cw, inv_cw;
+    int32_t dim[3];
+    int32_t bdim[3];
+    uint32_t cell_base, ncells;
+    // cell grid the reference would use this density round (sph.hpp:147-157)
+    double ocw[3];
+    int32_t onc[3];
+    double ocw_min;     // smallest cw over axes with more than one cell (+inf if none)
+};
+
+// Per-segment scalars updated by kernels.
+struct SegState {
+    unsigned long long bmin[3], bmax[3];   // ordered encodings of the particle bbox
+    unsigned long long dt_next;            // bits of the positive CFL step (atomicMin)
+    unsigned long long hmax[5];            // bits of max h at the start of rounds 0..4
+    double dt_cur, t;
+    long long steps;
+    int32_t active[4];
+    int32_t rounds;
+    int32_t bad;                           // non-zero: capacity or numeric trouble
+    double hmax_final;
+};
+
+// Device-side carried particle state, one buffer set.
+template <class R>
+struct Carried {
+    double *x, *y, *z, *vx, *vy, *vz, *u, *mass;
+    R *h, *D, *eps_act;
+    R* S;              // 6 x n, component-major {xx,yy,zz,xy,xz,yz}
+    uint32_t* info;
+    uint32_t* orig;    // IC index of the particle
+};
+
+// Vector record types for neighbour reads.
+template <class R> struct Vec4;
+template <> struct Vec4<float> { using T = float4; };
+template <> struct Vec4<double> { using T = double4; };
+
+// Everything the kernels read or write besides the carried state.
+template <class R>
+struct Work {
+    using V4 = typename Vec4<R>::T;
+    R *ax, *ay, *az;                 // acceleration from the last force pass
+    R *rho, *P, *cs, *csig, *Pf, *Rart, *dscale, *sigmax, *eps_work;
+    R *dudt, *dSdt;                  // from the last force pass (dSdt: 6 x n)
+    R *dudt_old, *dSdt_old;          // previous force pass, in the current order
+    V4* dens_rec;                    // (position key, mass)           density pass
+    V4* frec_pos;                    // (position key, h)              force pass
+    V4* frec_vel;                    // (velocity, mass)
+    V4* frec_th;                     // (rho, Pf/rho^2, csig, Rart)
+    V4* frec_sa;                     // S*dscale/rho^2 : xx yy zz xy
+    V4* frec_sb;                     // S*dscale/rho^2 : xz yz, m/rho, 0
+    uint32_t* cellc;                 // packed (cx | cy << 10 | cz << 20) of the sort cell
+    uint2* table;                    // per sort cell: [start, end) in the current order
+    const MatT<R>* mats;
+    const SegPar* par;
+    SegGrid* grid;
+    SegState* st;
+    int64_t n;
+    int32_t nseg;
+};
+
+// ---- small device helpers ---------------------------------------------------
+__device__ __forceinline__ unsigned long long ord_enc(double d) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(d);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __host__ __forceinline__ double ord_dec(unsigned long long e) {
+    unsigned long long b = (e & 0x8000000000000000ull) ? (e & 0x7fffffffffffffffull) : ~e;
+    double d;
+    __builtin_memcpy(&d, &b, 8);
+    return d;
+}
+__device__ __host__ __forceinline__ double bits_to_double(unsigned long long b) {
+    double d;
+    __builtin_memcpy(&d, &b, 8);
+    return d;
+}
+__device__ __forceinline__ unsigned long long double_to_bits(double d) {
+    return (unsigned long long)__double_as_longlong(d);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double wave_max_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_min_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Reduce `v` over the lanes of each segment present in the wave and apply one
+// atomic per (wave, segment).  Every lane of the wave must call this.
+// kind: 0 max, 1 min.  Values are encoded with `enc` (monotone uint64 map).
+template <int KIND, bool ORDERED>
+__device__ __forceinline__ void seg_atomic(unsigned long long* base, size_t stride_u64, int seg,
+                                           double v, bool valid) {
+    unsigned long long pending = __ballot(valid);
+    const int lane = lane_id();
+    while (pending) {
+        int leader = __builtin_ctzll(pending);
+        int lseg = __shfl(seg, leader);
+        bool mine = valid && seg == lseg && ((pending >> lane) & 1ull);
+        double x = mine ? v : (KIND == 0 ? -__builtin_huge_val() : __builtin_huge_val());
+        x = (KIND == 0) ? wave_max_d(x) : wave_min_d(x);
+        if (lane == leader) {
+            unsigned long long e = ORDERED ? ord_enc(x) : double_to_bits(x);
+            unsigned long long* p = base + (size_t)lseg * stride_u64;
+            if (KIND == 0) atomicMax(p, e);
+            else atomicMin(p, e);
+        }
+        pending &= ~__ballot(mine);
+    }
+}
+
+// Block index remap so consecutive tiles of the sorted particle range are
+// processed by workgroups of the same XCD (hardware dispatches workgroup b to
+// XCD b % 8); keeps a spatial neighbourhood inside one L2.
+__device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
+    if (!remap) return b;
+    const int nx = 8;
+    int q = nb / nx, r = nb % nx;
+    int x = b % nx, k = b / nx;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// Morton interleave of three 2-bit coordinates (within a 4x4x4 brick).
+__device__ __host__ __forceinline__ uint32_t brick_morton(uint32_t x, uint32_t y, uint32_t z) {
+    return (x & 1u) | ((y & 1u) << 1) | ((z & 1u) << 2) | ((x & 2u) << 2) | ((y & 2u) << 3) |
+           ((z & 2u) << 4);
+}
+__device__ __host__ __forceinline__ uint32_t cell_key(const SegGrid& g, int cx, int cy, int cz) {
+    uint32_t bl = ((uint32_t)(cx >> 2) * (uint32_t)g.bdim[1] + (uint32_t)(cy >> 2)) * (uint32_t)g.bdim[2] +
+                  (uint32_t)(cz >> 2);
+    return g.cell_base + bl * 64u + brick_morton((uint32_t)cx & 3u, (uint32_t)cy & 3u, (uint32_t)cz & 3u);
+}
+__device__ __host__ __forceinline__ uint32_t cellc_pack(int cx, int cy, int cz) {
+    return (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);
+}
+
+}  // namespace dsph

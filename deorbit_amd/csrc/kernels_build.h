@@ -1,0 +1,368 @@
+// Cell-list build: step control, kick + drift with a per-segment bounding box,
+// per-segment sort grid, cell keys, a stable LSD radix sort, and the
+// permutation of the carried state into cell order.
+//
+// Replaces the Metal counting sort (sph_force.metal:27-46 with the host scan of
+// gpu_ic.cpp:97-100) and the reference's per-round std::vector cell lists
+// (sph.hpp:147-162).  Every reduction is an exact min/max, and the sort is
+// stable, so a run is bitwise reproducible.
+#pragma once
+#include "common.h"
+
+namespace dsph {
+
+constexpr int kMaxStencil = 16;               // cap on the per-particle stencil radius in cells
+
+// ---------------------------------------------------------------- step control
+// mode 0: each segment takes its device CFL step; 1: host scalar dt; 2: per-segment host dt.
+__global__ void k_step_begin(SegState* st, const SegPar* par, int nseg, int mode, double host_dt,
+                             const double* seg_dt) {
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+        SegState& S = st[s];
+        double dt = (mode == 0) ? bits_to_double(S.dt_next) : (mode == 1 ? host_dt : seg_dt[s]);
+        // adaptive runs stop exactly at t_end (run_ic.cpp:327-330)
+        if (mode == 0 && par[s].t_end > 0.0 && S.t + dt > par[s].t_end) dt = par[s].t_end - S.t;
+        if (!(dt > 0.0)) {
+            if (!(dt == 0.0)) S.bad |= 2;
+            dt = 0.0;
+        }
+        S.dt_cur = dt;
+        S.t += dt;
+        S.steps += 1;
+    }
+}
+
+// Reset the per-build reductions (bbox, round maxima, round flags).
+__global__ void k_build_begin(SegState* st, int nseg) {
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+        SegState& S = st[s];
+        for (int a = 0; a < 3; a++) { S.bmin[a] = ~0ull; S.bmax[a] = 0ull; }
+        for (int r = 0; r < 5; r++) S.hmax[r] = 0ull;
+        for (int r = 0; r < 4; r++) S.active[r] = 0;
+        S.rounds = 0;
+        S.hmax_final = 0.0;
+    }
+}
+
+// Half kick and drift of non-fixed particles (sph.hpp:360-361, with the
+// gpu_ic gravity term of sph_force.metal:232-237), then the segment bbox over
+// finite positions.  move == 0 only measures the bbox.
+template <class R>
+__global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int move) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < w.n;
+    int seg = 0;
+    double x = 0, y = 0, z = 0;
+    bool fin = false;
+    if (in) {
+        const uint32_t inf = c.info[i];
+        seg = info_seg(inf);
+        x = c.x[i]; y = c.y[i]; z = c.z[i];
+        if (move && !info_fixed(inf)) {
+            const double dt = w.st[seg].dt_cur, half = 0.5 * dt;
+            const double g = w.par[seg].g;
+            double vx = c.vx[i] + (double)w.ax[i] * half;
+            double vy = c.vy[i] + (double)w.ay[i] * half;
+            double vz = c.vz[i] + (double)w.az[i] * half;
+            if (g != 0.0) vz -= g * half;
+            c.vx[i] = vx; c.vy[i] = vy; c.vz[i] = vz;
+            x += vx * dt; y += vy * dt; z += vz * dt;
+            c.x[i] = x; c.y[i] = y; c.z[i] = z;
+        }
+        fin = isfinite(x) && isfinite(y) && isfinite(z);
+        if (!fin) atomicOr(&w.st[seg].bad, 4);
+    }
+    const size_t stride = sizeof(SegState) / 8;
+    seg_atomic<1, true>(&w.st[0].bmin[0], stride, seg, x, fin);
+    seg_atomic<1, true>(&w.st[0].bmin[1], stride, seg, y, fin);
+    seg_atomic<1, true>(&w.st[0].bmin[2], stride, seg, z, fin);
+    seg_atomic<0, true>(&w.st[0].bmax[0], stride, seg, x, fin);
+    seg_atomic<0, true>(&w.st[0].bmax[1], stride, seg, y, fin);
+    seg_atomic<0, true>(&w.st[0].bmax[2], stride, seg, z, fin);
+}
+
+// Sort grid per segment: cubic cells of width cell_factor * h_init over the
+// segment bbox, padded to whole 4x4x4 bricks, laid end to end in one key
+// space.  If the total exceeds the table capacity every width grows by 25 %
+// until it fits (coarser cells stay correct: stencils are sized in cells).
+__global__ void k_grid(SegGrid* grid, SegState* st, const SegPar* par, int nseg, double cell_factor,
+                       uint32_t capacity, uint32_t* total_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double scale = 1.0;
+    unsigned long long total = 0;
+    for (int attempt = 0; attempt < 200; attempt++) {
+        total = 0;
+        for (int s = 0; s < nseg; s++) {
+            SegGrid& g = grid[s];
+            double lo[3], ext[3], emax = 0.0;
+            for (int a = 0; a < 3; a++) {
+                double l = ord_dec(st[s].bmin[a]), h = ord_dec(st[s].bmax[a]);
+                if (!(l <= h)) { l = 0.0; h = 0.0; }          // segment without finite particles
+                lo[a] = l;
+                ext[a] = h - l;
+                emax = ext[a] > emax ? ext[a] : emax;
+            }
+            double cw = cell_factor * par[s].h_init * scale;
+            if (emax / cw > 1000.0) cw = emax / 1000.0;         // at most 1001 cells per axis
+            // 64 cells per brick plus one trailing "park" cell that collects
+            // particles with a non-finite position (never visited by a stencil)
+            unsigned long long nc = 64;
+            for (int a = 0; a < 3; a++) {
+                g.lo[a] = lo[a];
+                g.dim[a] = (int)floor(ext[a] / cw) + 1;
+                g.bdim[a] = (g.dim[a] + 3) / 4;
+                nc *= (unsigned long long)g.bdim[a];
+            }
+            nc += 1;
+            g.cw = cw;
+            g.inv_cw = 1.0 / cw;
+            g.cell_base = (uint32_t)total;
+            g.ncells = (uint32_t)nc;
+            total += nc;
+        }
+        if (total <= capacity) break;
+        scale *= 1.25;
+    }
+    if (total > capacity) { st[0].bad |= 1; total = capacity; }
+    *total_out = (uint32_t)total;
+}
+
+__device__ __forceinline__ int cell_coord(const SegGrid& g, int a, double v) {
+    double t = floor((v - g.lo[a]) * g.inv_cw);
+    int c = (t < 0.0) ? 0 : (t > (double)(g.dim[a] - 1) ? g.dim[a] - 1 : (int)t);
+    return c;
+}
+
+// Cell key of every particle, and the clearing of the cell table (grid-stride).
+template <class R>
+__global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t* keys, uint32_t* vals,
+                                              const uint32_t* total_cells) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.n) {
+        const int seg = info_seg(c.info[i]);
+        const double x = c.x[i], y = c.y[i], z = c.z[i];
+        const SegGrid& g = w.grid[seg];
+        uint32_t key = g.cell_base + g.ncells - 1u;              // park cell
+        if (isfinite(x) && isfinite(y) && isfinite(z))
+            key = cell_key(g, cell_coord(g, 0, x), cell_coord(g, 1, y), cell_coord(g, 2, z));
+        keys[i] = key;
+        vals[i] = (uint32_t)i;
+    }
+    const uint32_t total = *total_cells;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < total; k += stride)
+        w.table[k] = make_uint2(0u, 0u);
+}
+
+// ---------------------------------------------------------------- radix sort
+// LSD radix sort of (key, value) pairs, 8-bit digits, 4096-key tiles.
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+// Lanes of the wave holding the same 8-bit digit as this lane.
+__device__ __forceinline__ unsigned long long match_digit(uint32_t d, bool valid) {
+    unsigned long long m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* keys, int64_t n, int shift,
+                                                       uint32_t* hist, int nb) {
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x, lane = lane_id();
+    h[t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int k = 0; k < kRsItems; k++) {
+        const int64_t idx = base + (int64_t)k * kRsThreads + t;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+        const unsigned long long m = match_digit(d, valid);
+        if (valid && __builtin_ctzll(m) == lane) atomicAdd(&h[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    hist[(size_t)t * nb + blockIdx.x] = h[t];
+}
+
+// Exclusive scan of m counters in place (one workgroup).
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* hist, int m) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (m + 1023) / 1024;
+    const int beg = t * per, end = min(m, beg + per);
+    uint32_t s = 0;
+    for (int i = beg; i < end; i++) s += hist[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (int i = beg; i < end; i++) {
+        uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin,
+                                                          uint32_t* kout, uint32_t* vout, int64_t n,
+                                                          int shift, const uint32_t* hist, int nb) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wc[4][256];
+    __shared__ uint32_t pre[4][256];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    base[t] = hist[(size_t)t * nb + blockIdx.x];
+    for (int q = 0; q < 4; q++) wc[q][t] = 0;
+    __syncthreads();
+    const int64_t tile = (int64_t)blockIdx.x * kRsTile;
+    for (int k = 0; k < kRsItems; k++) {
+        const int64_t idx = tile + (int64_t)k * kRsThreads + t;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t val = valid ? vin[idx] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        const unsigned long long m = match_digit(d, valid);
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (valid && __builtin_ctzll(m) == lane) wc[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        const uint32_t c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
+        pre[0][t] = 0;
+        pre[1][t] = c0;
+        pre[2][t] = c0 + c1;
+        pre[3][t] = c0 + c1 + c2;
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = base[d] + pre[wave][d] + rank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        base[t] += c0 + c1 + c2 + c3;
+        for (int q = 0; q < 4; q++) wc[q][t] = 0;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- permutation
+// Gather the carried state into sorted order, carry the last force pass's
+// dudt/dSdt along as the "old" derivatives (sph.hpp:358-359), record each
+// cell's [start, end) range, and write the density-pass neighbour records.
+// Positions in the mixed path are stored as FP32 offsets from the particle's
+// own sort-cell corner (pair separations are then formed from small numbers);
+// the FP64 path keeps absolute positions.
+template <class R>
+__global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst, Work<R> w,
+                                                 const uint32_t* keys, const uint32_t* vals) {
+    using V4 = typename Vec4<R>::T;
+    const int64_t n = w.n;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < n;
+    int seg = 0;
+    double hv = 0.0;
+    if (in) {
+        const uint32_t s = vals[i];
+        const uint32_t inf = src.info[s];
+        seg = info_seg(inf);
+        const double x = src.x[s], y = src.y[s], z = src.z[s], m = src.mass[s];
+        dst.x[i] = x; dst.y[i] = y; dst.z[i] = z;
+        dst.vx[i] = src.vx[s]; dst.vy[i] = src.vy[s]; dst.vz[i] = src.vz[s];
+        dst.u[i] = src.u[s]; dst.mass[i] = m;
+        const R h = src.h[s];
+        dst.h[i] = h;
+        dst.D[i] = src.D[s];
+        dst.eps_act[i] = src.eps_act[s];
+#pragma unroll
+        for (int k = 0; k < 6; k++) dst.S[(size_t)k * n + i] = src.S[(size_t)k * n + s];
+        dst.info[i] = inf;
+        dst.orig[i] = src.orig[s];
+        w.dudt_old[i] = w.dudt[s];
+#pragma unroll
+        for (int k = 0; k < 6; k++) w.dSdt_old[(size_t)k * n + i] = w.dSdt[(size_t)k * n + s];
+
+        const uint32_t key = keys[i];
+        const uint32_t prev = (i > 0) ? keys[i - 1] : 0xffffffffu;
+        const uint32_t next = (i + 1 < n) ? keys[i + 1] : 0xffffffffu;
+        if (key != prev) w.table[key].x = (uint32_t)i;
+        if (key != next) w.table[key].y = (uint32_t)(i + 1);
+        const SegGrid& g = w.grid[seg];
+        if (key != g.cell_base + g.ncells - 1u) {
+            const int cx = cell_coord(g, 0, x), cy = cell_coord(g, 1, y), cz = cell_coord(g, 2, z);
+            w.cellc[i] = cellc_pack(cx, cy, cz);
+            V4 rec;
+            if (sizeof(R) == 4) {
+                rec.x = (R)(x - (g.lo[0] + (double)cx * g.cw));
+                rec.y = (R)(y - (g.lo[1] + (double)cy * g.cw));
+                rec.z = (R)(z - (g.lo[2] + (double)cz * g.cw));
+            } else {
+                rec.x = (R)x; rec.y = (R)y; rec.z = (R)z;
+            }
+            rec.w = (R)m;
+            w.dens_rec[i] = rec;
+        } else {
+            w.cellc[i] = 0u;
+            V4 rec;
+            rec.x = rec.y = rec.z = (R)__builtin_nan("");
+            rec.w = (R)0;
+            w.dens_rec[i] = rec;
+        }
+        hv = (double)h;
+    }
+    seg_atomic<0, false>(&w.st[0].hmax[0], sizeof(SegState) / 8, seg, hv, in);
+}
+
+// Density round bookkeeping, one thread per segment (sph.hpp:185-206).
+// r in 0..3: decide whether round r runs and derive the cell grid the
+// reference would build for it; r == 4: record the rounds run and the final
+// h_max for the force stencils.
+__global__ void k_round(SegState* st, SegGrid* grid, const SegPar* par, int nseg, int r) {
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+        SegState& S = st[s];
+        if (r < 4) {
+            int act = 1;
+            if (r > 0) {
+                const double hm = bits_to_double(S.hmax[r]), h0 = bits_to_double(S.hmax[r - 1]);
+                act = S.active[r - 1] && !(hm <= h0 * 1.02);
+            }
+            S.active[r] = act;
+            if (act) {
+                SegGrid& g = grid[s];
+                const double hmax = fmax(bits_to_double(S.hmax[r]), 1e-30);
+                const double rc = 2.0 * hmax;
+                double cmin = __builtin_huge_val();
+                for (int a = 0; a < 3; a++) {
+                    double ext = par[s].dhi[a] - par[s].dlo[a];
+                    if (ext <= 0) ext = rc;
+                    int m = (int)(ext / rc);
+                    m = m < 1 ? 1 : m;
+                    if (par[s].per[a] && m < 3) m = 1;
+                    g.onc[a] = m;
+                    g.ocw[a] = ext / m;
+                    if (m > 1 && g.ocw[a] < cmin) cmin = g.ocw[a];
+                }
+                g.ocw_min = cmin;
+            }
+        } else {
+            if (!S.active[0]) {
+                S.rounds = 0;
+                S.hmax_final = bits_to_double(S.hmax[0]);
+            } else {
+                int last = 3;
+                while (last > 0 && !S.active[last]) last--;
+                S.rounds = last + 1;
+                S.hmax_final = bits_to_double(S.hmax[last + 1]);
+            }
+        }
+    }
+}
+
+}  // namespace dsph

@@ -1,0 +1,215 @@
+"""Seeded initial conditions for the SPH impact workloads.
+
+Every generator returns an IC dict with float64 ``pos (n,3)``, ``vel (n,3)``,
+``mass``, ``u``, int32 ``mat``, uint8 ``fixed`` plus ``dx``, the domain box
+``lo``/``hi`` (the reference's ``set_domain`` bounds, sph.hpp:96-101, and the
+header of the IC file, run_ic.cpp:136-159), the solver ``params``, the
+``materials`` list and a ``meta`` dict.  The target is written first and the
+impactor last, as the reference IC builders do (boundary_daemon.py:119-158), so
+that Weibull draws (which follow particle index order, sph.hpp:117-128) are
+reproducible from the file alone.
+
+Geometry conventions: lattice spacing ``dx``; the target's top surface is the
+plane z = 0 with particles at cell centres below it; the impactor is the set of
+integer lattice points i^2+j^2+k^2 <= R^2 (R in lattice units), its lowest point
+one spacing above the surface, moving along +x and down at ``theta`` degrees.
+Contact happens about one third of the way along the target in x.  Each
+coordinate is jittered by U(-jitter*dx, +jitter*dx) from a PCG64 stream.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import materials as M
+
+
+def sphere_offsets(R: int) -> np.ndarray:
+    """Integer lattice points with i^2 + j^2 + k^2 <= R^2, in (i, j, k) lexicographic order."""
+    r = np.arange(-R, R + 1)
+    i, j, k = np.meshgrid(r, r, r, indexing="ij")
+    sel = (i * i + j * j + k * k) <= R * R
+    return np.stack([i[sel], j[sel], k[sel]], axis=1).astype(np.float64)
+
+
+def slab_centres(nx: int, ny: int, nz: int) -> np.ndarray:
+    """Cell centres of an nx x ny x nz block below z = 0, (i, j, k) lexicographic."""
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    return np.stack([i.ravel() + 0.5, j.ravel() + 0.5, -(k.ravel() + 0.5)], axis=1).astype(np.float64)
+
+
+def _finish(pos, vel, mass, mat, dx, params, mats, meta, margin=4.0, fixed=None):
+    n = len(mass)
+    lo = pos.min(axis=0) - margin * dx
+    hi = pos.max(axis=0) + margin * dx
+    prm = dict(params)
+    prm["lo"] = tuple(float(v) for v in lo)
+    prm["hi"] = tuple(float(v) for v in hi)
+    return dict(pos=np.ascontiguousarray(pos), vel=np.ascontiguousarray(vel),
+                mass=np.ascontiguousarray(mass), u=np.zeros(n), mat=mat.astype(np.int32),
+                fixed=np.zeros(n, np.uint8) if fixed is None else fixed.astype(np.uint8),
+                dx=float(dx), lo=prm["lo"], hi=prm["hi"], params=prm, materials=mats, meta=meta)
+
+
+def impact(R: int, theta_deg: float, speed: float, nx: int, ny: int, layers, dx: float = 10.0,
+           jitter: float = 0.01, seed: int = 1000, strength: bool = False, frac: bool = False,
+           relief=None, porous_top: tuple | None = None, name: str = "impact", steps: int = 100):
+    """Sphere on a layered target.
+
+    ``layers`` is a list of ``(material index, thickness in cells)`` from the top
+    down.  ``relief`` (optional, nx x ny, in cells) raises/lowers each column's top.
+    ``porous_top = (thickness_cells, density_factor)`` scales the particle mass of
+    the top layer (there is no porosity model; only the mass changes).
+    """
+    mats = M.impact_materials(strength)
+    nz = int(sum(t for _, t in layers))
+    # --- target, column by column in (i, j, k) order.  Column (i, j) fills the
+    # cells between the flat bottom z = -nz and its local top z = rel[i, j].
+    rel = (np.zeros((nx, ny), np.int64) if relief is None
+           else np.rint(np.asarray(relief)).astype(np.int64))
+    rmax = int(max(0, rel.max()))
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz + rmax), indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    zc = rmax - (kk + 0.5)
+    col = rel[ii, jj]
+    keep = zc < col
+    k_t = (col - rmax + kk)[keep]                              # 0 = top cell of the column
+    pos_t = np.stack([ii[keep] + 0.5, jj[keep] + 0.5, zc[keep]], axis=1).astype(np.float64)
+    mat_t = np.full(len(pos_t), layers[-1][0], np.int32)
+    edge = 0
+    for m, t in layers:
+        mat_t[(k_t >= edge) & (k_t < edge + t)] = m
+        edge += t
+    rho0 = np.array([m["rho0"] for m in mats])
+    mass_t = rho0[mat_t] * dx ** 3
+    if porous_top is not None:
+        thick, fac = porous_top
+        mass_t[k_t < thick] *= fac
+    # --- impactor
+    off = sphere_offsets(R)
+    th = math.radians(theta_deg)
+    lx, ly = nx, ny
+    x_contact = lx / 3.0
+    gap = 1.0
+    xc = x_contact - gap / math.tan(th)
+    zc = gap + R + (0.0 if relief is None else float(np.max(relief)))
+    yc = ly / 2.0
+    pos_i = off + np.array([xc, yc, zc])
+    n_t, n_i = len(pos_t), len(pos_i)
+    pos = np.concatenate([pos_t, pos_i]) * dx
+    vel = np.zeros_like(pos)
+    vel[n_t:, 0] = speed * math.cos(th)
+    vel[n_t:, 2] = -speed * math.sin(th)
+    mass = np.concatenate([mass_t, np.full(n_i, mats[M.MAT_IMPACTOR]["rho0"] * dx ** 3)])
+    mat = np.concatenate([mat_t, np.full(n_i, M.MAT_IMPACTOR, np.int32)])
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pos += rng.uniform(-jitter * dx, jitter * dx, size=pos.shape)
+    params = M.default_params(dx, (0, 0, 0), (1, 1, 1), frac)
+    meta = dict(name=name, n_target=n_t, n_impactor=n_i, R=R, theta=theta_deg, speed=speed,
+                nx=nx, ny=ny, nz=nz, seed=seed, steps=steps, strength=strength, frac=frac)
+    return _finish(pos, vel, mass, mat, dx, params, mats, meta)
+
+
+# ------------------------------------------------------------------ workloads
+def config0():
+    """Plain FP64 CPU-runnable case: R=10dx iron sphere at 45 deg, 8 km/s, onto a
+    64x32x16 granite slab; no strength; 100 steps."""
+    return impact(10, 45.0, 8000.0, 64, 32, [(M.MAT_TARGET, 16)], seed=1000, name="config0",
+                  steps=100)
+
+
+def config1():
+    """~1.05M particles: R=40dx at 3 deg, 8 km/s, into a 128x64x96 granite slab;
+    strength (iron Y=1 GPa, G=80 GPa; granite Y=2 GPa, G=30 GPa) and Weibull damage
+    (k=1e39 m^-3, m=9) on; 500 steps."""
+    return impact(40, 3.0, 8000.0, 128, 64, [(M.MAT_TARGET, 96)], seed=1001, strength=True,
+                  frac=True, name="config1", steps=500)
+
+
+def config2():
+    """~4.0M particles: R=60dx at 2 deg, 8 km/s, into 192x96 columns of 32 water
+    cells over 136 granite cells; strength and damage on; 1000 steps."""
+    return impact(60, 2.0, 8000.0, 192, 96, [(M.MAT_WATER, 32), (M.MAT_TARGET, 136)],
+                  seed=1002, strength=True, frac=True, name="config2", steps=1000)
+
+
+def gaussian_relief(nx, ny, amplitude, corr, seed):
+    """Gaussian random surface: white noise filtered by a Gaussian of correlation
+    length ``corr`` (cells), scaled to standard deviation ``amplitude`` (cells)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = rng.standard_normal((nx, ny))
+    kx = np.fft.fftfreq(nx)[:, None]
+    ky = np.fft.fftfreq(ny)[None, :]
+    filt = np.exp(-0.5 * (2 * math.pi) ** 2 * (kx ** 2 + ky ** 2) * corr ** 2)
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * filt))
+    f -= f.mean()
+    sd = f.std()
+    return f * (amplitude / sd if sd > 0 else 0.0)
+
+
+def config4():
+    """~32M particles: R=120dx at 1.5 deg, 8 km/s, onto a 384x192 granite half-space
+    ~339 cells deep with a 16-cell top layer at 0.98 rho0 (mass-scaled) and a
+    Gaussian surface relief (4dx amplitude, 30dx correlation, seed 4000); 200 steps."""
+    rel = gaussian_relief(384, 192, 4.0, 30.0, 4000)
+    return impact(120, 1.5, 8000.0, 384, 192, [(M.MAT_TARGET, 339)], seed=4001, strength=True,
+                  frac=True, relief=rel, porous_top=(16, 0.98), name="config4", steps=200)
+
+
+def ensemble_members(count: int = 256, seed0: int = 3000, nmin: int = 8000, nmax: int = 200000):
+    """Parameters of the batched ensemble: per instance N ~ logU[nmin, nmax], angle
+    U[1,45] deg, speed U[4,12] km/s, impactor/target particle ratio U[0.05,0.3].
+    The target is a 2:1:1 slab; the impactor radius follows from its particle count."""
+    out = []
+    for i in range(count):
+        rng = np.random.Generator(np.random.PCG64(seed0 + i))
+        n = float(np.exp(rng.uniform(math.log(nmin), math.log(nmax))))
+        ang = float(rng.uniform(1.0, 45.0))
+        spd = float(rng.uniform(4000.0, 12000.0))
+        ratio = float(rng.uniform(0.05, 0.3))
+        n_t = n / (1.0 + ratio)
+        n_i = n - n_t
+        ny = max(4, int(round((n_t / 2.0) ** (1.0 / 3.0))))
+        R = max(2, int(round((3.0 * n_i / (4.0 * math.pi)) ** (1.0 / 3.0))))
+        out.append(dict(index=i, seed=seed0 + i, R=R, theta=ang, speed=spd, nx=2 * ny, ny=ny,
+                        nz=ny, ratio=ratio))
+    return out
+
+
+def ensemble_instance(member: dict, steps: int = 500):
+    return impact(member["R"], member["theta"], member["speed"], member["nx"], member["ny"],
+                  [(M.MAT_TARGET, member["nz"])], seed=member["seed"], strength=True, frac=True,
+                  name=f"ensemble{member['index']:03d}", steps=steps)
+
+
+def tiny_impact(R=4, nx=16, ny=10, nz=6, theta=30.0, speed=6000.0, strength=True, frac=True,
+                seed=7, jitter=0.01):
+    """A few-thousand-particle impact with the full physics, for fast parity tests."""
+    return impact(R, theta, speed, nx, ny, [(M.MAT_TARGET, nz)], seed=seed, strength=strength,
+                  frac=frac, jitter=jitter, name="tiny", steps=20)
+
+
+def sheared_cube(L: int = 22, dx: float = 30.0):
+    """The reference's force-oracle state builder (cpu_dump.cpp:11-31): an L^3 basalt
+    cube with a central iron sphere of radius 5dx, shear/convergence velocities and
+    an x-gradient in u; fracture physics on.  Materials: 0 basalt, 1 iron."""
+    lo = -L * dx / 2.0
+    hi = L * dx / 2.0
+    i, j, k = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    x = lo + (i.ravel() + 0.5) * dx
+    y = lo + (j.ravel() + 0.5) * dx
+    z = lo + (k.ravel() + 0.5) * dx
+    R = 5.0 * dx
+    iron = (x * x + y * y + z * z) < R * R
+    pos = np.stack([x, y, z], axis=1)
+    vel = np.stack([80.0 * (y / hi), np.zeros_like(x), -40.0 * (z / hi)], axis=1)
+    u = 2.0e5 * (1.0 + x / hi)
+    rho0 = np.where(iron, 7800.0, 2700.0)
+    mass = rho0 * dx * dx * dx
+    mat = iron.astype(np.int32)
+    mats = [M.basalt_preset(), M.iron_preset()]
+    params = M.default_params(dx, (lo - dx,) * 3, (hi + dx,) * 3, True)
+    return dict(pos=pos, vel=vel, mass=mass, u=u, mat=mat, fixed=np.zeros(len(x), np.uint8),
+                dx=dx, lo=params["lo"], hi=params["hi"], params=params, materials=mats,
+                meta=dict(name="sheared_cube", steps=12, seed=None))

@@ -1,0 +1,238 @@
+"""ctypes binding of the CPU FP64 oracle C API (sph_oracle.h).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and bench.py's
+cpu_baseline leg.  Two interchangeable libraries implement the API:
+
+* ``libsph_oracle.so``  -- this project's restatement (sph_oracle.cpp);
+* ``_ref/libsph_ref.so`` -- the reference solver's own headers
+  (hydrocodes/sph/sph.hpp) compiled in place, present only where the reference
+  tree was available at build time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libsph_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libsph_ref.so")
+
+POS, VEL, ACC, MASS, RHO, U, DUDT, P, CS, CSIG, HH, EPS_WORK, D, EPS_ACT, SIGMAX, PF, RART, \
+    DSCALE, S, DSDT, MAT, FIXED = range(22)
+FIELD_NAMES = ["pos", "vel", "acc", "mass", "rho", "u", "dudt", "P", "cs", "csig", "hh",
+               "eps_work", "D", "eps_act", "sigmax", "Pf", "Rart", "dscale", "S", "dSdt",
+               "mat", "fixed"]
+WIDTH = {POS: 3, VEL: 3, ACC: 3, S: 6, DSDT: 6}
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("gamma", C.c_double),
+                ("rho0", C.c_double), ("A", C.c_double), ("B", C.c_double), ("a", C.c_double),
+                ("b", C.c_double), ("alpha", C.c_double), ("beta", C.c_double), ("u0", C.c_double),
+                ("uiv", C.c_double), ("ucv", C.c_double), ("G", C.c_double), ("Y", C.c_double),
+                ("wm", C.c_double), ("wk", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("eta", C.c_double), ("h_init", C.c_double), ("alpha", C.c_double),
+                ("beta", C.c_double), ("eps", C.c_double), ("alpha_u", C.c_double),
+                ("cfl", C.c_double), ("eps_as", C.c_double), ("damage_on", C.c_int32),
+                ("periodic", C.c_int32 * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("nthreads", C.c_int32)]
+
+
+def material_struct(m: dict) -> Material:
+    """Build the C record from a material dict (deorbit_amd.materials layout)."""
+    s = Material()
+    s.type = 1 if m.get("type", "tillotson") == "tillotson" else 0
+    s.gamma = m.get("gamma", 1.4)
+    for k in ("rho0", "A", "B", "a", "b", "alpha", "beta", "u0", "uiv", "ucv", "G", "Y", "wm", "wk"):
+        setattr(s, k, float(m.get(k, 0.0)))
+    return s
+
+
+def params_struct(p: dict, nthreads: int = 0) -> Params:
+    s = Params()
+    for k in ("eta", "h_init", "alpha", "beta", "eps", "alpha_u", "cfl", "eps_as"):
+        setattr(s, k, float(p[k]))
+    s.damage_on = int(bool(p.get("damage_on", False)))
+    per = p.get("periodic", (0, 0, 0))
+    for a in range(3):
+        s.periodic[a] = int(bool(per[a]))
+        s.lo[a] = float(p["lo"][a])
+        s.hi[a] = float(p["hi"][a])
+    s.nthreads = int(nthreads)
+    return s
+
+
+_LIBS: dict[str, C.CDLL] = {}
+
+
+def load(which: str = "oracle") -> C.CDLL:
+    """Load ``"oracle"`` (the restatement) or ``"ref"`` (reference headers)."""
+    path = ORACLE_SO if which == "oracle" else REF_SO
+    if path in _LIBS:
+        return _LIBS[path]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (run `make -C oracle` / `make -C oracle ref`)")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    lib.orc_create.restype = C.c_void_p
+    lib.orc_create.argtypes = [C.POINTER(Params), C.POINTER(Material), C.c_int32]
+    lib.orc_destroy.argtypes = [C.c_void_p]
+    lib.orc_add.argtypes = [C.c_void_p, C.c_int64, dp, dp, dp, dp, C.POINTER(C.c_int32),
+                            C.POINTER(C.c_uint8)]
+    lib.orc_n.restype = C.c_int64
+    lib.orc_n.argtypes = [C.c_void_p]
+    lib.orc_seed_damage.argtypes = [C.c_void_p, C.c_double, C.c_uint32]
+    for fn in ("orc_init", "orc_compute_forces", "orc_compute_strain_stress"):
+        getattr(lib, fn).argtypes = [C.c_void_p]
+    lib.orc_timestep.restype = C.c_double
+    lib.orc_timestep.argtypes = [C.c_void_p]
+    lib.orc_step.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_grow_damage.argtypes = [C.c_void_p, C.c_double]
+    lib.orc_compute_density_h.argtypes = [C.c_void_p, C.c_int32]
+    lib.orc_yield_limit.argtypes = [C.c_void_p, C.c_int64]
+    lib.orc_get.restype = C.c_int32
+    lib.orc_get.argtypes = [C.c_void_p, C.c_int32, dp]
+    lib.orc_set.restype = C.c_int32
+    lib.orc_set.argtypes = [C.c_void_p, C.c_int32, dp]
+    lib.orc_cell_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), dp, dp]
+    for fn in ("orc_kW", "orc_kdWdr"):
+        getattr(lib, fn).restype = C.c_double
+        getattr(lib, fn).argtypes = [C.c_double, C.c_double]
+    lib.orc_eig_max.restype = C.c_double
+    lib.orc_eig_max.argtypes = [dp]
+    lib.orc_material_emod.restype = C.c_double
+    lib.orc_material_emod.argtypes = [C.POINTER(Material)]
+    for fn in ("orc_pressure", "orc_sound_speed"):
+        getattr(lib, fn).restype = C.c_double
+        getattr(lib, fn).argtypes = [C.POINTER(Material), C.c_double, C.c_double]
+    _LIBS[path] = lib
+    return lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class System:
+    """Python handle mirroring the reference ``struct System`` (sph.hpp:72-380)."""
+
+    def __init__(self, params: dict, materials: list[dict], which: str = "oracle",
+                 nthreads: int = 0):
+        self.lib = load(which)
+        self._mats = (Material * len(materials))(*[material_struct(m) for m in materials])
+        self._prm = params_struct(params, nthreads)
+        self.h = self.lib.orc_create(C.byref(self._prm), self._mats, len(materials))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.orc_destroy(h)
+            self.h = None
+
+    @property
+    def n(self) -> int:
+        return int(self.lib.orc_n(self.h))
+
+    def add(self, pos, vel, mass, u, mat, fixed=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        mat = np.ascontiguousarray(mat, dtype=np.int32)
+        n = len(mass)
+        fx = np.zeros(n, np.uint8) if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        self.lib.orc_add(self.h, n, _dp(pos), _dp(vel), _dp(mass), _dp(u),
+                         mat.ctypes.data_as(C.POINTER(C.c_int32)),
+                         fx.ctypes.data_as(C.POINTER(C.c_uint8)))
+
+    def add_ic(self, ic: dict):
+        self.add(ic["pos"], ic["vel"], ic["mass"], ic["u"], ic["mat"], ic["fixed"])
+
+    def seed_damage(self, dx: float, seed: int = 12345):
+        self.lib.orc_seed_damage(self.h, dx, seed)
+
+    def init(self):
+        self.lib.orc_init(self.h)
+
+    def timestep(self) -> float:
+        return float(self.lib.orc_timestep(self.h))
+
+    def step(self, dt: float):
+        self.lib.orc_step(self.h, dt)
+
+    def compute_density_h(self, iters: int = 2):
+        self.lib.orc_compute_density_h(self.h, iters)
+
+    def compute_forces(self):
+        self.lib.orc_compute_forces(self.h)
+
+    def compute_strain_stress(self):
+        self.lib.orc_compute_strain_stress(self.h)
+
+    def grow_damage(self, dt: float):
+        self.lib.orc_grow_damage(self.h, dt)
+
+    def yield_limit(self, i: int):
+        self.lib.orc_yield_limit(self.h, i)
+
+    def get(self, field: int) -> np.ndarray:
+        w = WIDTH.get(field, 1)
+        out = np.empty(self.n * w, np.float64)
+        if self.lib.orc_get(self.h, field, _dp(out)) != 0:
+            raise ValueError(f"bad field {field}")
+        return out.reshape(-1, w) if w > 1 else out
+
+    def set(self, field: int, values):
+        w = WIDTH.get(field, 1)
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != self.n * w:
+            raise ValueError("size mismatch")
+        if self.lib.orc_set(self.h, field, _dp(a)) != 0:
+            raise ValueError(f"bad field {field}")
+
+    def cell_info(self):
+        nc = (C.c_int32 * 3)()
+        cw = (C.c_double * 3)()
+        hm = C.c_double()
+        self.lib.orc_cell_info(self.h, nc, cw, C.byref(hm))
+        return tuple(nc), tuple(cw), hm.value
+
+    def state(self) -> dict:
+        return {name: self.get(k) for k, name in enumerate(FIELD_NAMES)}
+
+
+def make_system(ic: dict, params: dict, materials: list[dict], which: str = "oracle",
+                nthreads: int = 0, seed_damage: int | None = None, dx: float | None = None) -> System:
+    s = System(params, materials, which, nthreads)
+    s.add_ic(ic)
+    if seed_damage is not None:
+        s.seed_damage(dx if dx is not None else ic["dx"], seed_damage)
+    return s
+
+
+def eos(which: str, m: dict, rho: float, u: float):
+    lib = load(which)
+    ms = material_struct(m)
+    return lib.orc_pressure(C.byref(ms), rho, u), lib.orc_sound_speed(C.byref(ms), rho, u)
+
+
+def emod(which: str, m: dict) -> float:
+    return load(which).orc_material_emod(C.byref(material_struct(m)))
+
+
+def eig_max(which: str, s6) -> float:
+    a = np.ascontiguousarray(s6, dtype=np.float64)
+    return load(which).orc_eig_max(_dp(a))
+
+
+def kW(which: str, r: float, h: float) -> float:
+    return load(which).orc_kW(r, h)
+
+
+def kdWdr(which: str, r: float, h: float) -> float:
+    return load(which).orc_kdWdr(r, h)
