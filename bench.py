@@ -1,0 +1,166 @@
+"""Throughput of the MI355X SPH impact step (one JSON line on stdout).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu]
+
+A step is one System::step of every particle of the workload (kick/drift, cell
+sort, adaptive-h density, EOS, strain + forces, damage, finish), with the CFL
+time step kept on the device.  The default workload is the ~1.05M-particle
+grazing impact (iron sphere R = 40 dx at 3 deg and 8 km/s into a 128x64x96
+granite slab, strength and Weibull damage on), mixed precision.  Inputs are
+resident in HBM before the timed region; K steps are replayed from a captured
+hipGraph and bracketed by device synchronisation.
+
+Besides the contract fields the line carries:
+  roofline     -- dominant kernel's algorithmic bytes / its mean HIP-event time
+                  (a profiled pass over the same workload right after the timed one);
+  cpu_baseline -- the reference solver's own code (oracle/_ref, compiled from the
+                  reference headers) timed on this host on a bounded sample;
+  fp64         -- the all-FP64 device path on the same workload and its ratio.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from deorbit_amd import dsph, ics, roofline  # noqa: E402
+
+CONFIGS = {0: ics.config0, 1: ics.config1, 2: ics.config2, 4: ics.config4}
+WORKLOAD = {
+    0: "config0: R=10dx iron sphere at 45 deg, 8 km/s, onto a 64x32x16 granite slab, no strength",
+    1: "config1: R=40dx iron sphere at 3 deg, 8 km/s, into a 128x64x96 granite slab, strength+damage",
+    2: "config2: R=60dx iron sphere at 2 deg, 8 km/s, into 192x96 columns of 32 water over 136 granite",
+    4: "config4: R=120dx iron sphere at 1.5 deg, 8 km/s, onto a 384x192x339 granite half-space",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_run(sysm, steps, graph=True):
+    sysm.synchronize()
+    t0 = time.perf_counter()
+    sysm.run(steps, 0.0, graph)
+    sysm.synchronize()
+    return time.perf_counter() - t0
+
+
+def gpu_leg(ic, precision, warmup, steps):
+    s = dsph.make_system(ic, precision)
+    s.init()
+    s.run(warmup, 0.0, True)
+    el = timed_run(s, steps, True)
+    return s, el
+
+
+def profile_leg(s, steps):
+    s.set_profiling(True)
+    s.reset_profile()
+    s.run(steps, 0.0, False)
+    s.synchronize()
+    prof = s.profile()
+    s.set_profiling(False)
+    return prof
+
+
+def cpu_baseline(ic, seconds_budget=25.0):
+    """Reference solver on the host: init, then as many steps as fit the budget (>= 1)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import pyoracle as po
+    kind = "reference" if os.path.exists(po.REF_SO) else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    o = po.make_system(ic, ic["params"], ic["materials"], which,
+                       seed_damage=12345 if ic["params"]["damage_on"] else None, dx=ic["dx"])
+    t0 = time.perf_counter()
+    o.init()
+    t_init = time.perf_counter() - t0
+    times = []
+    while True:
+        dt = o.timestep()
+        t0 = time.perf_counter()
+        o.step(dt)
+        times.append(time.perf_counter() - t0)
+        if sum(times) + t_init > seconds_budget or len(times) >= 20:
+            break
+    med = float(np.median(times))
+    cores = min(16, os.cpu_count() or 1)     # parallel_for caps at 16 threads (sph.hpp:58-60)
+    return {"value": len(ic["mass"]) / med, "unit": "particle-steps/s", "cores": cores, "kind": kind,
+            "sample": f"{len(times)} step(s) of the same workload after init, median {med * 1e3:.0f} ms/step "
+                      f"(init {t_init:.1f} s excluded); std::thread parallel_for over {cores} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: the SPH step has no multi-GPU path in this build "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    ic = CONFIGS[args.config]()
+    n = len(ic["mass"])
+    log(f"workload {ic['meta']['name']}: n={n}")
+
+    s, el = gpu_leg(ic, dsph.MIXED, args.warmup, args.steps)
+    value = n * args.steps / el
+    ms = 1e3 * el / args.steps
+    log(f"mixed: {ms:.3f} ms/step  {value / 1e6:.1f} M particle-steps/s")
+    prof = profile_leg(s, args.profile_steps)
+    for k, (tot, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        if cnt:
+            log(f"  {k:11s} {tot / args.profile_steps:8.3f} ms/step  ({cnt} launches)")
+    kb = roofline.kernel_bytes("mixed")
+    force_ms = prof["force"][0] / max(1, prof["force"][1])
+    achieved = kb["force"] * n / (force_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_force<float>", "achieved": round(achieved, 2),
+            "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
+            "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5), "traffic": None,
+            "bytes_per_particle": kb["force"], "mean_launch_ms": round(force_ms, 4),
+            "step_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5)}
+    diag = s.diagnostics()
+    s.close()
+
+    fp64 = None
+    if not args.no_fp64:
+        s64, el64 = gpu_leg(ic, dsph.FP64, min(args.warmup, 5), max(5, args.steps // 4))
+        v64 = n * max(5, args.steps // 4) / el64
+        s64.close()
+        fp64 = {"value": v64, "ms_per_step": 1e3 * el64 / max(5, args.steps // 4),
+                "mixed_over_fp64": value / v64}
+        log(f"fp64: {fp64['ms_per_step']:.3f} ms/step  ratio {fp64['mixed_over_fp64']:.2f}x")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ic)
+        log(f"cpu: {cpu['value'] / 1e6:.3f} M particle-steps/s ({cpu['kind']}, {cpu['cores']} threads)")
+
+    line = {
+        "metric": "SPH particle-steps/s on 1 MI355X, mixed vs FP64 at stated error; runs/hr",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 pair/EOS/stress, f64 state+reductions",
+        "data": "synthetic (seeded lattice impact IC)",
+        "config": {"workload": WORKLOAD[args.config], "particles": n, "dt": "device CFL (cfl=0.2)",
+                   "graph": True, "t_sim_end": diag["t"]},
+        "roofline": roof, "cpu_baseline": cpu, "fp64": fp64,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

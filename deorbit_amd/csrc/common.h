@@ -70,6 +70,8 @@ template <class R> struct Vec4;
 template <> struct Vec4<float> { using T = float4; };
 template <> struct Vec4<double> { using T = double4; };
 
+struct BlockPart;
+
 // Everything the kernels read or write besides the carried state.
 template <class R>
 struct Work {
@@ -86,6 +88,8 @@ struct Work {
     V4* frec_sb;                     // S*dscale/rho^2 : xz yz, m/rho, 0
     uint32_t* cellc;                 // packed (cx | cy << 10 | cz << 20) of the sort cell
     uint2* table;                    // per sort cell: [start, end) in the current order
+    float* cell_hmax;                // per sort cell: upper bound of h over its particles
+    BlockPart* parts;                // per-block partial reductions
     const MatT<R>* mats;
     const SegPar* par;
     SegGrid* grid;
@@ -146,6 +150,123 @@ __device__ __forceinline__ void seg_atomic(unsigned long long* base, size_t stri
             else atomicMin(p, e);
         }
         pending &= ~__ballot(mine);
+    }
+}
+
+// ---- block-level segmented reductions ---------------------------------------
+// A reduction value k of segment s lands in *(base[k] + s * stride) through
+// atomicMin/atomicMax on a monotone 64-bit encoding (ordered encoding for
+// signed values, raw IEEE bits for non-negative ones).  Blocks whose valid
+// lanes all belong to one segment write a partial record instead of issuing
+// atomics; k_parts_reduce folds the records (one atomic per run of equal
+// segments per thread), which keeps same-address atomics to a few hundred per
+// launch.
+constexpr int kMaxRed = 6;
+struct BlockPart {
+    int32_t seg;          // -1: no record (empty block or handled by atomics)
+    int32_t pad;
+    double v[kMaxRed];
+};
+template <int K>
+struct SegTargets {
+    unsigned long long* base[K];
+    int32_t is_max[K];
+    int32_t ordered[K];
+};
+
+__device__ __forceinline__ unsigned long long red_enc(double x, int ordered) {
+    return ordered ? ord_enc(x) : double_to_bits(x);
+}
+__device__ __forceinline__ void red_apply(unsigned long long* p, double x, int is_max, int ordered) {
+    const unsigned long long e = red_enc(x, ordered);
+    if (is_max) atomicMax(p, e);
+    else atomicMin(p, e);
+}
+
+// Every thread of the block must call this (it synchronises the block).
+template <int K>
+__device__ __forceinline__ void block_reduce(const SegTargets<K>& t, size_t stride, BlockPart* parts, int seg,
+                                             const double (&v)[K], bool valid) {
+    __shared__ int s_lo, s_hi;
+    __shared__ double s_part[K][16];
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nwave = (blockDim.x + 63) >> 6;
+    if (threadIdx.x == 0) { s_lo = 0x7fffffff; s_hi = -1; }
+    __syncthreads();
+    {
+        int lo = valid ? seg : 0x7fffffff, hi = valid ? seg : -1;
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+        }
+        if (lane == 0) { atomicMin(&s_lo, lo); atomicMax(&s_hi, hi); }
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
+    if (hi < 0) {                                   // no valid lane in the block
+        if (threadIdx.x == 0) parts[blockIdx.x].seg = -1;
+        __syncthreads();
+        return;
+    }
+    if (lo == hi) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double x = valid ? v[k] : (t.is_max[k] ? -__builtin_huge_val() : __builtin_huge_val());
+            x = t.is_max[k] ? wave_max_d(x) : wave_min_d(x);
+            if (lane == 0) s_part[k][wave] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            BlockPart bp;
+            bp.seg = lo;
+            bp.pad = 0;
+            for (int k = 0; k < K; k++) {
+                double x = s_part[k][0];
+                for (int q = 1; q < nwave; q++) x = t.is_max[k] ? fmax(x, s_part[k][q]) : fmin(x, s_part[k][q]);
+                bp.v[k] = x;
+            }
+            parts[blockIdx.x] = bp;
+        }
+        __syncthreads();
+        return;
+    }
+    // several segments in this block: per-wave, per-segment atomics
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int lseg = __shfl(seg, leader);
+        const bool mine = valid && seg == lseg && ((pending >> lane) & 1ull);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double x = mine ? v[k] : (t.is_max[k] ? -__builtin_huge_val() : __builtin_huge_val());
+            x = t.is_max[k] ? wave_max_d(x) : wave_min_d(x);
+            if (lane == leader) red_apply(t.base[k] + (size_t)lseg * stride, x, t.is_max[k], t.ordered[k]);
+        }
+        pending &= ~__ballot(mine);
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x].seg = -1;
+    __syncthreads();
+}
+
+template <int K>
+__global__ void k_parts_reduce(SegTargets<K> t, size_t stride, const BlockPart* parts, int nparts) {
+    const int nt = blockDim.x * gridDim.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = (nparts + nt - 1) / nt;
+    const int beg = tid * per, end = min(nparts, beg + per);
+    int cur = -1;
+    double acc[K];
+    for (int q = beg; q <= end; q++) {
+        const int s = (q < end) ? parts[q].seg : -2;
+        if (s == -1) continue;
+        if (s != cur) {
+            if (cur >= 0)
+                for (int k = 0; k < K; k++) red_apply(t.base[k] + (size_t)cur * stride, acc[k], t.is_max[k], t.ordered[k]);
+            cur = s;
+            if (s >= 0)
+                for (int k = 0; k < K; k++) acc[k] = parts[q].v[k];
+        } else {
+            for (int k = 0; k < K; k++) acc[k] = t.is_max[k] ? fmax(acc[k], parts[q].v[k]) : fmin(acc[k], parts[q].v[k]);
+        }
     }
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -75,7 +76,7 @@ class Engine : public EngineBase {
         nseg_ = (int)segp.size();
         tune_.block_pair = 256;
         tune_.block_point = 256;
-        tune_.cell_factor = 1.0;
+        tune_.cell_factor = 1.5;
         tune_.xcd_remap = 1;
         tune_.cell_capacity_per_particle = 4;
         for (auto& e : prof_total_) e = 0.0;
@@ -137,6 +138,8 @@ class Engine : public EngineBase {
         w.frec_pos = dalloc<V4>(n_); w.frec_vel = dalloc<V4>(n_); w.frec_th = dalloc<V4>(n_);
         w.frec_sa = dalloc<V4>(n_); w.frec_sb = dalloc<V4>(n_);
         w.cellc = dalloc<uint32_t>(n_);
+        nparts_max_ = (int)blocks_for(n_, 64) + 1;
+        w.parts = dalloc<BlockPart>(nparts_max_);
         alloc_table();
         // sort buffers
         nb_sort_ = (int)blocks_for(n_, kRsTile);
@@ -214,6 +217,7 @@ class Engine : public EngineBase {
         cap = std::min<int64_t>(cap, (int64_t)0xfffffff0);
         capacity_ = (uint32_t)cap;
         w_.table = dalloc<uint2>(capacity_);
+        w_.cell_hmax = dalloc<float>(capacity_);
         int bits = 1;
         while (bits < 32 && ((1ull << bits) <= (unsigned long long)capacity_)) bits++;
         passes_ = (bits + 7) / 8;
@@ -309,13 +313,47 @@ class Engine : public EngineBase {
         }
     }
 
+    // ---------------------------------------------------------------- reductions
+    unsigned long long* st_field(size_t offset) { return (unsigned long long*)((char*)d_st_ + offset); }
+    SegTargets<6> bbox_targets() {
+        SegTargets<6> t;
+        for (int a = 0; a < 3; a++) {
+            t.base[a] = st_field(offsetof(SegState, bmin) + 8 * a);
+            t.base[3 + a] = st_field(offsetof(SegState, bmax) + 8 * a);
+            t.is_max[a] = 0; t.is_max[3 + a] = 1;
+            t.ordered[a] = 1; t.ordered[3 + a] = 1;
+        }
+        return t;
+    }
+    SegTargets<1> hmax_target(int slot) {
+        SegTargets<1> t;
+        t.base[0] = st_field(offsetof(SegState, hmax) + 8 * slot);
+        t.is_max[0] = 1;
+        t.ordered[0] = 0;
+        return t;
+    }
+    SegTargets<1> dt_target() {
+        SegTargets<1> t;
+        t.base[0] = st_field(offsetof(SegState, dt_next));
+        t.is_max[0] = 0;
+        t.ordered[0] = 0;
+        return t;
+    }
+    template <int K>
+    void fold_parts(int phase, const SegTargets<K>& t, unsigned nblocks) {
+        launch(phase, "parts_reduce", k_parts_reduce<K>, dim3(1), dim3(256), t, sizeof(SegState) / 8,
+               (const BlockPart*)w_.parts, (int)nblocks);
+    }
+
     // ---------------------------------------------------------------- phases
     Carried<R>& cur() { return bufs_[cur_]; }
 
-    void build(int move) {
+    void build(int move, bool carry_all) {
         const int bp = tune_.block_point;
         launch(PH_ROUND, "build_begin", k_build_begin, dim3(1), dim3(256), d_st_, nseg_);
-        launch(PH_KICK_DRIFT, "kick_drift", k_kick_drift<R>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_, move);
+        const auto tb = bbox_targets();
+        launch(PH_KICK_DRIFT, "kick_drift", k_kick_drift<R>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_, move, tb);
+        fold_parts(PH_KICK_DRIFT, tb, blocks_for(n_, bp));
         launch(PH_GRID, "grid", k_grid, dim3(1), dim3(64), d_grid_, d_st_, (const SegPar*)d_par_, nseg_,
                tune_.cell_factor, capacity_, d_total_cells_);
         launch(PH_KEYS, "keys", k_keys<R>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_, keys_[0], vals_[0],
@@ -331,11 +369,29 @@ class Engine : public EngineBase {
                    nb_sort_);
             ka = 1 - ka;
         }
+        const auto th = hmax_target(0);
         launch(PH_PERMUTE, "permute", k_permute<R>, dim3(blocks_for(n_, bp)), dim3(bp), bufs_[cur_],
-               bufs_[1 - cur_], w_, (const uint32_t*)keys_[ka], (const uint32_t*)vals_[ka]);
+               bufs_[1 - cur_], w_, (const uint32_t*)keys_[ka], (const uint32_t*)vals_[ka], th);
+        fold_parts(PH_PERMUTE, th, blocks_for(n_, bp));
         cur_ = 1 - cur_;
         built_ = true;
         records_ = false;
+        if (carry_all) {
+            // Outside a step every per-particle field is state the caller may read
+            // or feed back (System keeps them between calls), so carry them through
+            // the re-sort as well.
+            const uint32_t* perm = vals_[ka];
+            R* scratch = (R*)stage_;
+            Work<R>& w = w_;
+            for (R* f : {w.ax, w.ay, w.az, w.rho, w.P, w.cs, w.csig, w.Pf, w.Rart, w.dscale, w.sigmax,
+                         w.eps_work}) {
+                launch(PH_PERMUTE, "gather", k_gather<R>, dim3(blocks_for(n_, bp)), dim3(bp), (const R*)f, scratch,
+                       perm, n_);
+                HIPC(hipMemcpyAsync(f, scratch, n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+            }
+            HIPC(hipMemcpyAsync(w.dudt, w.dudt_old, n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+            HIPC(hipMemcpyAsync(w.dSdt, w.dSdt_old, 6 * n_ * sizeof(R), hipMemcpyDeviceToDevice, stream_));
+        }
     }
 
     void density() {
@@ -343,8 +399,10 @@ class Engine : public EngineBase {
         const unsigned nb = blocks_for(n_, bq);
         for (int r = 0; r < 4; r++) {
             launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, r);
-            launch(PH_DENSITY, "density", k_density<R, false>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap);
-            launch(PH_DENSITY, "density", k_density<R, true>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap);
+            const auto th = hmax_target(r + 1);
+            launch(PH_DENSITY, "density", k_density<R, false>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap, th);
+            launch(PH_DENSITY, "density", k_density<R, true>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap, th);
+            fold_parts(PH_DENSITY, th, nb);
         }
         launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, 4);
         records_ = true;
@@ -356,7 +414,10 @@ class Engine : public EngineBase {
 
     void force() {
         const int bq = tune_.block_pair;
-        launch(PH_FORCE, "force", k_force<R>, dim3(blocks_for(n_, bq)), dim3(bq), cur(), w_, tune_.xcd_remap);
+        const auto td = dt_target();
+        launch(PH_ROUND, "reset_dt", k_reset_dt, dim3(1), dim3(256), d_st_, nseg_);
+        launch(PH_FORCE, "force", k_force<R>, dim3(blocks_for(n_, bq)), dim3(bq), cur(), w_, tune_.xcd_remap, td);
+        fold_parts(PH_FORCE, td, blocks_for(n_, bq));
         have_dt_ = true;
     }
 
@@ -369,7 +430,7 @@ class Engine : public EngineBase {
         if (mode == 0 && !have_dt_) throw Err{DSPH_E_STATE, "no device time step yet: call dsph_init first"};
         launch(PH_STEP_BEGIN, "step_begin", k_step_begin, dim3(1), dim3(256), d_st_, (const SegPar*)d_par_, nseg_,
                mode, dt, (const double*)d_segdt_);
-        build(1);
+        build(1, false);
         density();
         force();
         finish(1, 1);
@@ -377,17 +438,17 @@ class Engine : public EngineBase {
 
     // ---------------------------------------------------------------- API ops
     void init() {
-        build(0);
+        build(0, false);
         density();
         force();
     }
     void compute_density_h() {
-        build(0);
+        build(0, true);
         density();
     }
     void compute_forces() {
         if (!built_) {
-            build(0);
+            build(0, true);
             finalize_rounds();
         }
         if (!records_) {
@@ -680,8 +741,10 @@ class Engine : public EngineBase {
                 tune_ = nt;
                 // the table is reallocated; everything else is kept
                 release_graphs();
-                auto it = std::find(allocs_.begin(), allocs_.end(), (void*)w_.table);
-                if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+                for (void* old : {(void*)w_.table, (void*)w_.cell_hmax}) {
+                    auto it = std::find(allocs_.begin(), allocs_.end(), old);
+                    if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
+                }
                 alloc_table();
                 built_ = false;
                 records_ = false;
@@ -820,7 +883,7 @@ class Engine : public EngineBase {
     uint32_t* vals_[2] = {nullptr, nullptr};
     uint32_t* hist_ = nullptr;
     uint32_t* d_total_cells_ = nullptr;
-    int nb_sort_ = 0, passes_ = 0;
+    int nb_sort_ = 0, passes_ = 0, nparts_max_ = 0;
     uint32_t capacity_ = 0;
     MatT<R>* d_mats_ = nullptr;
     SegPar* d_par_ = nullptr;
@@ -979,16 +1042,21 @@ int dsph_upload(dsph_ctx* c, int64_t n, const double* pos3, const double* vel3, 
         const int nmat = (int)c->mats.size();
         int last_seg = hp.seg.empty() ? 0 : hp.seg.back();
         const double h_init_default = c->segp[0].h_init;
-        for (int64_t i = 0; i < n; i++) {
+        require((int64_t)hp.mass.size() + n <= (int64_t)0x7fffffff, DSPH_E_RANGE, "more than 2^31-1 particles");
+        for (int64_t i = 0; i < n; i++) {           // validate everything before appending anything
             const int s = seg ? seg[i] : 0;
             require(s >= 0 && s < nseg, DSPH_E_ARG, "segment index out of range");
             require(s >= last_seg, DSPH_E_ARG, "segment indices must be non-decreasing in IC order");
             last_seg = s;
             require(mat[i] >= 0 && mat[i] < nmat, DSPH_E_ARG, "material index out of range");
             require(mass[i] > 0 && std::isfinite(mass[i]), DSPH_E_ARG, "masses must be positive and finite");
+            require(std::isfinite(u[i]), DSPH_E_ARG, "internal energies must be finite");
             for (int a = 0; a < 3; a++)
                 require(std::isfinite(pos3[3 * i + a]) && std::isfinite(vel3[3 * i + a]), DSPH_E_ARG,
                         "positions and velocities must be finite");
+        }
+        for (int64_t i = 0; i < n; i++) {
+            const int s = seg ? seg[i] : 0;
             for (int a = 0; a < 3; a++) { hp.pos.push_back(pos3[3 * i + a]); hp.vel.push_back(vel3[3 * i + a]); }
             hp.mass.push_back(mass[i]);
             hp.u.push_back(u[i]);

@@ -32,6 +32,11 @@ __global__ void k_step_begin(SegState* st, const SegPar* par, int nseg, int mode
     }
 }
 
+// Clear the CFL minimum before a force pass recomputes it.
+__global__ void k_reset_dt(SegState* st, int nseg) {
+    for (int s = threadIdx.x; s < nseg; s += blockDim.x) st[s].dt_next = 0x7ff0000000000000ull;
+}
+
 // Reset the per-build reductions (bbox, round maxima, round flags).
 __global__ void k_build_begin(SegState* st, int nseg) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
@@ -48,7 +53,7 @@ __global__ void k_build_begin(SegState* st, int nseg) {
 // gpu_ic gravity term of sph_force.metal:232-237), then the segment bbox over
 // finite positions.  move == 0 only measures the bbox.
 template <class R>
-__global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int move) {
+__global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int move, SegTargets<6> tg) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < w.n;
     int seg = 0;
@@ -72,13 +77,8 @@ __global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int
         fin = isfinite(x) && isfinite(y) && isfinite(z);
         if (!fin) atomicOr(&w.st[seg].bad, 4);
     }
-    const size_t stride = sizeof(SegState) / 8;
-    seg_atomic<1, true>(&w.st[0].bmin[0], stride, seg, x, fin);
-    seg_atomic<1, true>(&w.st[0].bmin[1], stride, seg, y, fin);
-    seg_atomic<1, true>(&w.st[0].bmin[2], stride, seg, z, fin);
-    seg_atomic<0, true>(&w.st[0].bmax[0], stride, seg, x, fin);
-    seg_atomic<0, true>(&w.st[0].bmax[1], stride, seg, y, fin);
-    seg_atomic<0, true>(&w.st[0].bmax[2], stride, seg, z, fin);
+    const double v[6] = {x, y, z, x, y, z};
+    block_reduce<6>(tg, sizeof(SegState) / 8, w.parts, seg, v, fin);
 }
 
 // Sort grid per segment: cubic cells of width cell_factor * h_init over the
@@ -150,8 +150,10 @@ __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t*
     }
     const uint32_t total = *total_cells;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < total; k += stride)
+    for (uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < total; k += stride) {
         w.table[k] = make_uint2(0u, 0u);
+        w.cell_hmax[k] = 0.0f;
+    }
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, 
 // the FP64 path keeps absolute positions.
 template <class R>
 __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst, Work<R> w,
-                                                 const uint32_t* keys, const uint32_t* vals) {
+                                                 const uint32_t* keys, const uint32_t* vals, SegTargets<1> tg) {
     using V4 = typename Vec4<R>::T;
     const int64_t n = w.n;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -317,7 +319,8 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
         }
         hv = (double)h;
     }
-    seg_atomic<0, false>(&w.st[0].hmax[0], sizeof(SegState) / 8, seg, hv, in);
+    const double v[1] = {hv};
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, in);
 }
 
 // Density round bookkeeping, one thread per segment (sph.hpp:185-206).

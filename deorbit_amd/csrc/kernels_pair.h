@@ -105,6 +105,15 @@ __device__ __forceinline__ void eos_epilogue(const Carried<R>& c, const Work<R>&
     const R sc = dsc * ir2;
     sa.x = s6[0] * sc; sa.y = s6[1] * sc; sa.z = s6[2] * sc; sa.w = s6[3] * sc;
     sb.x = s6[4] * sc; sb.y = s6[5] * sc; sb.z = mi / rho; sb.w = R(0);
+    // per-cell upper bound of h for the force stencil (positive floats order as integers)
+    const uint32_t cc = w.cellc[i];
+    const SegGrid& g = w.grid[info_seg(c.info[i])];
+    if (isfinite((double)ox)) {
+        float hf = (float)h;
+        if ((double)hf < (double)h) hf = __uint_as_float(__float_as_uint(hf) + 1u);   // round up
+        atomicMax((unsigned int*)&w.cell_hmax[cell_key(g, cc & 1023u, (cc >> 10) & 1023u, cc >> 20)],
+                  __float_as_uint(hf));
+    }
     w.frec_pos[i] = a;
     w.frec_vel[i] = b;
     w.frec_th[i] = t;
@@ -118,7 +127,7 @@ __device__ __forceinline__ void eos_epilogue(const Carried<R>& c, const Work<R>&
 // LAST marks the second iteration of the round: it also reduces max h for the
 // round test and runs the EOS epilogue.
 template <class R, bool LAST>
-__global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r, int remap) {
+__global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r, int remap, SegTargets<1> tg) {
     using V4 = typename Vec4<R>::T;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
@@ -205,7 +214,10 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
         hnew_d = (double)hn;
         if (LAST) eos_epilogue<R>(c, w, i, p, info_mat(c.info[i]), rho, hn, ox, oy, oz);
     }
-    if (LAST) seg_atomic<0, false>(&w.st[0].hmax[r + 1], sizeof(SegState) / 8, seg, hnew_d, act);
+    if (LAST) {
+        const double v[1] = {hnew_d};
+        block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
+    }
 }
 
 // EOS epilogue on its own (System::compute_forces entry without a density pass).
@@ -238,7 +250,7 @@ __device__ __forceinline__ double f_sqrt(double x) { return sqrt(x); }
 //   a     += m_j (S_i d_i/rho_i^2 + S_j d_j/rho_j^2) . grad W
 // then the Jaumann stress rate, the deviatoric work and the CFL step.
 template <class R>
-__global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap) {
+__global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, SegTargets<1> tg) {
     using V4 = typename Vec4<R>::T;
     const int64_t n = w.n;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -293,8 +305,12 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                         const int nz = cz + dz;
                         if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
                         const R gz0 = cell_gap(dz, oz, cw);
-                        if (gxy + gz0 * gz0 >= rad2) continue;
-                        const uint2 rg = w.table[cell_key(g, nx, ny, nz)];
+                        const R gg = gxy + gz0 * gz0;
+                        if (gg >= rad2) continue;
+                        const uint32_t key = cell_key(g, nx, ny, nz);
+                        const R reach = hi + (R)w.cell_hmax[key];
+                        if (gg >= reach * reach) continue;
+                        const uint2 rg = w.table[key];
                         R bx, by, bz;
                         if (sizeof(R) == 4) {
                             bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
@@ -406,7 +422,8 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
         dt_i = p.cfl * (double)hi / ((double)csig_i * (1.0 + p.alpha) + 1e-30);
         dt_valid = true;
     }
-    seg_atomic<1, false>(&w.st[0].dt_next, sizeof(SegState) / 8, seg, dt_i, dt_valid);
+    const double v[1] = {dt_i};
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, dt_valid);
 }
 
 }  // namespace dsph

@@ -148,6 +148,13 @@ __global__ void k_diag_final(const DiagPart* parts, const int32_t* seg_chunk0, i
     }
 }
 
+// dst[k] = src[perm[k]]: carries a per-particle field through a re-sort.
+template <class T>
+__global__ void k_gather(const T* src, T* dst, const uint32_t* perm, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[perm[k]];
+}
+
 // ---------------------------------------------------------------- IC-order I/O
 template <class T>
 __global__ void k_export(const T* src, double* dst, int width, int comp, const uint32_t* orig, int64_t n) {

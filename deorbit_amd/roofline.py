@@ -1,0 +1,53 @@
+"""Algorithmic HBM bytes per particle for each kernel of the step.
+
+Counting rules: every per-particle array a kernel touches is counted once per
+launch (neighbour records are credited once, as if every re-read were served
+from cache), cell-table reads are ignored, and the sort counts only its key and
+value traffic.  ``achieved GB/s = bytes_per_particle * n / kernel time``.
+The figures follow the data layout of deorbit_amd/csrc/common.h:
+
+mixed (R = float): positions, velocities, u and mass in FP64; everything else
+FP32; neighbour records are float4.  fp64 (R = double): every R array doubles
+and records are double4.
+"""
+from __future__ import annotations
+
+MI355X_HBM_GBPS = 8000.0          # published peak HBM3E bandwidth
+MI355X_FP32_TFLOPS = 157.3        # published dense FP32 (packed) vector peak
+MI355X_FP64_TFLOPS = 78.6
+
+
+def kernel_bytes(precision: str = "mixed") -> dict:
+    r = 4 if precision == "mixed" else 8         # bytes of the pair/state real R
+    rec = 4 * r                                  # one neighbour record (4 x R)
+    d = 8
+    out = {}
+    # kick + drift + bbox: R info, x y z, v xyz, acc xyz; W v xyz, x y z
+    out["kick_drift"] = 4 + 6 * d + 3 * r + 6 * d
+    # keys: R info, x y z; W key, value
+    out["keys"] = 4 + 3 * d + 8
+    # radix sort, per pass: hist reads keys; scatter reads and writes key+value
+    out["sort_pass"] = 4 + 8 + 8
+    # permute: carried state R+W (8 doubles, h D eps_act, S6, info, orig) + keys/vals R
+    #          + old derivatives R+W (dudt, dSdt6) + density record W + cellc W
+    carried = 8 * d + 3 * r + 6 * r + 4 + 4
+    out["permute"] = 2 * carried + 8 + 2 * 7 * r + rec + 4
+    # density iteration: neighbour record once + own (h, cellc, info, mass) R; W rho, h
+    out["density"] = rec + r + 4 + 4 + d + 2 * r
+    # last iteration adds the EOS epilogue: R u, S6, D, vel; W P cs csig Pf dscale Rart sigmax + 5 records
+    out["density_last"] = out["density"] + (d + 7 * r + 3 * d) + (7 * r + 5 * rec)
+    # force: 5 neighbour records once + own (cellc, info, S6, dscale) R; W acc3 dudt dSdt6 eps_work
+    out["force"] = 5 * rec + 4 + 4 + 6 * r + r + (3 * r + r + 6 * r + r)
+    if precision != "mixed":
+        out["force"] += 3 * d                    # own absolute position for the cell offset
+    # finish: R info, sigmax, D, eps_act, rho, v3, acc3, u, dudt_old, dudt, S6, dSdt_old6, dSdt6
+    #         W D, v3, u, S6
+    out["finish"] = (4 + 5 * r + 3 * d + 3 * r + d + 2 * r + 18 * r) + (r + 3 * d + d + 6 * r)
+    return out
+
+
+def step_bytes(precision: str = "mixed", passes: int = 3, iterations: int = 2) -> int:
+    """Algorithmic bytes of one full step (one density round of `iterations`)."""
+    k = kernel_bytes(precision)
+    return (k["kick_drift"] + k["keys"] + passes * k["sort_pass"] + k["permute"]
+            + (iterations - 1) * k["density"] + k["density_last"] + k["force"] + k["finish"])

@@ -87,7 +87,7 @@ def load(which: str = "oracle") -> C.CDLL:
     lib.orc_n.restype = C.c_int64
     lib.orc_n.argtypes = [C.c_void_p]
     lib.orc_seed_damage.argtypes = [C.c_void_p, C.c_double, C.c_uint32]
-    for fn in ("orc_init", "orc_compute_forces", "orc_compute_strain_stress"):
+    for fn in ("orc_init", "orc_compute_forces", "orc_compute_strain_stress", "orc_build_cells"):
         getattr(lib, fn).argtypes = [C.c_void_p]
     lib.orc_timestep.restype = C.c_double
     lib.orc_timestep.argtypes = [C.c_void_p]
@@ -164,6 +164,9 @@ class System:
 
     def step(self, dt: float):
         self.lib.orc_step(self.h, dt)
+
+    def build_cells(self):
+        self.lib.orc_build_cells(self.h)
 
     def compute_density_h(self, iters: int = 2):
         self.lib.orc_compute_density_h(self.h, iters)

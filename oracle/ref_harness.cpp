@@ -53,6 +53,7 @@ void orc_seed_damage(orc_sys* s, double dx, uint32_t seed) { s->sys.seed_damage(
 void orc_init(orc_sys* s) { s->sys.init(); }
 double orc_timestep(orc_sys* s) { return s->sys.timestep(); }
 void orc_step(orc_sys* s, double dt) { s->sys.step(dt); }
+void orc_build_cells(orc_sys* s) { s->sys.build_cells(); }
 void orc_compute_density_h(orc_sys* s, int32_t iters) { s->sys.compute_density_h(iters); }
 void orc_compute_strain_stress(orc_sys* s) { s->sys.compute_strain_stress(); }
 void orc_compute_forces(orc_sys* s) { s->sys.compute_forces(); }

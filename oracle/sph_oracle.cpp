@@ -530,6 +530,7 @@ void orc_seed_damage(orc_sys* s, double dx, uint32_t seed) {        // sph.hpp:1
 void orc_init(orc_sys* s) { s->density_h(2); s->forces(); }
 double orc_timestep(orc_sys* s) { return s->cfl_dt(); }
 void orc_step(orc_sys* s, double dt) { s->advance(dt); }
+void orc_build_cells(orc_sys* s) { s->build_cells(); }
 void orc_compute_density_h(orc_sys* s, int32_t iters) { s->density_h(iters); }
 void orc_compute_strain_stress(orc_sys* s) { s->strain_stress(); }
 void orc_compute_forces(orc_sys* s) { s->forces(); }

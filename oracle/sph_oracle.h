@@ -57,6 +57,7 @@ void     orc_seed_damage(orc_sys* s, double dx, uint32_t seed);       /* sph.hpp
 void     orc_init(orc_sys* s);                                       /* sph.hpp:379     */
 double   orc_timestep(orc_sys* s);                                   /* sph.hpp:350-355 */
 void     orc_step(orc_sys* s, double dt);                            /* sph.hpp:357-377 */
+void     orc_build_cells(orc_sys* s);                                /* sph.hpp:147-162 */
 void     orc_compute_density_h(orc_sys* s, int32_t iters);           /* sph.hpp:184-208 */
 void     orc_compute_strain_stress(orc_sys* s);                      /* sph.hpp:212-258 */
 void     orc_compute_forces(orc_sys* s);                             /* sph.hpp:260-318 */

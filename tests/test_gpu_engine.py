@@ -1,0 +1,201 @@
+"""Engine properties of the device step: determinism, graph replay, batching,
+size-independent invariants at the benchmark sizes, and error handling."""
+import numpy as np
+import pytest
+
+from conftest import scale_err
+from deorbit_amd import dsph, ics
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "u", "rho", "hh", "S", "D", "acc", "dudt")
+
+
+def run_states(ic, precision, steps, graph, tuning=None):
+    s = dsph.make_system(ic, precision)
+    if tuning:
+        s.set_tuning(**tuning)
+    s.init()
+    s.run(steps, 0.0, graph)
+    st = s.state(list(FIELDS))
+    d = s.diagnostics()
+    s.close()
+    return st, d
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_runs_are_bitwise_reproducible(precision, gpu):
+    ic = ics.tiny_impact()
+    a, _ = run_states(ic, precision, 6, False)
+    b, _ = run_states(ic, precision, 6, False)
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_graph_replay_equals_stream_launches(precision, gpu):
+    # graphs are replayed several times, with eager work and synchronisation in between
+    ic = ics.tiny_impact()
+    a, da = run_states(ic, precision, 7, False)
+    s = dsph.make_system(ic, precision)
+    s.init()
+    for _ in range(3):
+        s.run(2, 0.0, True)            # one replay of the two-step graph
+        s.diagnostics()                # eager kernels + host synchronisation between replays
+    s.run(1, 0.0, True)                # odd remainder: one stream-launched step
+    b = s.state(list(FIELDS))
+    db = s.diagnostics()
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+    assert da["t"] == db["t"] and da["steps"] == db["steps"] == 7
+
+
+def test_tuning_changes_speed_not_results_beyond_roundoff(gpu):
+    ic = ics.tiny_impact()
+    a, _ = run_states(ic, dsph.FP64, 4, True)
+    b, _ = run_states(ic, dsph.FP64, 4, True, dict(cell_factor=1.0, block_pair=64))
+    for f in FIELDS:
+        assert scale_err(b[f], a[f]) < 1e-12, f
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_batch_of_segments_equals_separate_runs(precision, gpu):
+    """Three impacts of uneven size in one batched context step exactly like three
+    separate contexts (each segment keeps its own grid, dt and density rounds)."""
+    members = [ics.tiny_impact(seed=1), ics.tiny_impact(R=3, nx=12, ny=8, nz=5, theta=60, seed=2),
+               ics.tiny_impact(R=5, nx=18, ny=12, nz=7, theta=15, speed=9000, seed=3)]
+    sep = []
+    for ic in members:
+        s = dsph.make_system(ic, precision)
+        s.init()
+        s.run(5, 0.0, False)
+        sep.append((s.state(list(FIELDS)), s.diagnostics()))
+        s.close()
+    b = dsph.System([ic["params"] for ic in members], members[0]["materials"], precision)
+    for k, ic in enumerate(members):
+        b.add_ic(ic, seg=np.full(len(ic["mass"]), k, np.int32))
+    for k, ic in enumerate(members):
+        b.seed_damage(ic["dx"], 12345, seg=k)
+    b.init()
+    b.run(5, 0.0, True)
+    st = b.state(list(FIELDS))
+    off = 0
+    for k, ic in enumerate(members):
+        n = len(ic["mass"])
+        for f in FIELDS:
+            assert np.array_equal(st[f][off:off + n], sep[k][0][f]), (k, f)
+        d = b.diagnostics(k)
+        assert d["t"] == sep[k][1]["t"] and d["e_tot"] == sep[k][1]["e_tot"]
+        off += n
+    dts = b.timestep()
+    assert len(dts) == 3 and len(set(dts.tolist())) == 3       # each segment its own CFL step
+
+
+def test_full_size_invariants_config1(gpu):
+    """~1.05M particles with strength and damage: after init and after 10 steps the
+    pairwise momentum exchange cancels (sum m a = 0) and h stays in its clamp."""
+    ic = ics.config1()
+    for precision in (dsph.MIXED, dsph.FP64):
+        s = dsph.make_system(ic, precision)
+        s.init()
+        for k in range(2):
+            m = s.get("mass")
+            ma = m[:, None] * s.get("acc")
+            tol = 1e-10 if precision == dsph.FP64 else 2e-5
+            assert np.all(np.abs(ma.sum(0)) <= tol * np.abs(ma).sum(0)), (precision, k)
+            h = s.get("hh")
+            h0 = ic["params"]["h_init"]
+            assert h.min() >= 0.5 * h0 * (1 - 1e-6) and h.max() <= 2 * h0 * (1 + 1e-6)
+            d = s.diagnostics()
+            assert d["nonfinite"] == 0 and d["n"] == len(m)
+            s.run(10, 0.0, True)
+        s.close()
+
+
+def test_full_size_fp64_init_matches_oracle_config1(gpu):
+    ic = ics.config1()
+    o = po.make_system(ic, ic["params"], ic["materials"], seed_damage=12345, dx=ic["dx"])
+    o.init()
+    s = dsph.make_system(ic, dsph.FP64)
+    s.init()
+    for name, f in (("rho", po.RHO), ("hh", po.HH), ("P", po.P), ("acc", po.ACC), ("dudt", po.DUDT),
+                    ("dSdt", po.DSDT), ("sigmax", po.SIGMAX)):
+        e = scale_err(s.get(name), o.get(f))
+        print(f"config1 init fp64 {name}={e:.2e}", flush=True)
+        assert e < 1e-12, name
+    assert abs(s.timestep() - o.timestep()) <= 1e-14 * o.timestep()
+
+
+def test_energy_rate_balance_config0_mixed(gpu):
+    # no strength, no artificial stress: sum m (v.a + du/dt) vanishes pairwise
+    s = dsph.make_system(ics.config0(), dsph.MIXED)
+    s.init()
+    m, v, a, du = s.get("mass"), s.get("vel"), s.get("acc"), s.get("dudt")
+    kin = m * np.sum(v * a, axis=1)
+    th = m * du
+    assert abs(kin.sum() + th.sum()) <= 1e-5 * (np.abs(kin).sum() + np.abs(th).sum())
+
+
+def test_argument_errors_are_reported(gpu):
+    ic = ics.tiny_impact()
+    p = dict(ic["params"])
+    with pytest.raises(dsph.DsphError) as e:
+        dsph.System(dict(p, eta=-1.0), ic["materials"])
+    assert e.value.code == -1
+    with pytest.raises(dsph.DsphError):
+        dsph.System(dict(p, periodic=(True, False, False)), ic["materials"])
+    s = dsph.System(p, ic["materials"])
+    bad = np.array(ic["mat"])
+    bad[3] = 7
+    with pytest.raises(dsph.DsphError):
+        s.add(ic["pos"], ic["vel"], ic["mass"], ic["u"], bad)
+    s.add_ic(ic)
+    with pytest.raises(dsph.DsphError) as e:
+        s.step(0.0)                    # no device dt before init
+    assert e.value.code == -2
+    with pytest.raises(ValueError):
+        s.set("rho", np.zeros(3))
+    s.init()
+    s.step(0.0)
+    s.synchronize()
+
+
+def test_nonfinite_particle_is_parked_and_reported(gpu):
+    ic = ics.tiny_impact()
+    s = dsph.make_system(ic, dsph.MIXED)
+    s.init()
+    pos = s.get("pos")
+    pos[5] = np.nan
+    s.set("pos", pos)
+    s.step(1e-6)                       # must finish: the particle sits in its segment's park cell
+    with pytest.raises(dsph.DsphError) as e:
+        s.synchronize()
+    assert e.value.code == -6
+    d = s.diagnostics()
+    assert d["nonfinite"] >= 1 and d["n"] == len(ic["mass"])
+
+
+def test_autotune_keeps_state(gpu):
+    ic = ics.tiny_impact()
+    s = dsph.make_system(ic, dsph.FP64)
+    s.init()
+    before = s.state(list(FIELDS))
+    t = s.autotune(2)
+    assert t["block_pair"] in (64, 128, 256) and 1.0 <= t["cell_factor"] <= 2.0
+    after = s.state(list(FIELDS))
+    for f in FIELDS:
+        assert np.array_equal(before[f], after[f]), f
+    s.run(2, 0.0, True)
+    s.synchronize()
+
+
+def test_profile_reports_every_phase(gpu):
+    s = dsph.make_system(ics.tiny_impact(), dsph.MIXED)
+    s.init()
+    s.set_profiling(True)
+    s.reset_profile()
+    s.run(3, 0.0, False)
+    p = s.profile()
+    for ph in ("kick_drift", "sort", "permute", "density", "force", "finish"):
+        assert p[ph][1] >= 3 and p[ph][0] > 0, ph
