@@ -247,26 +247,47 @@ __device__ __forceinline__ void block_reduce(const SegTargets<K>& t, size_t stri
     __syncthreads();
 }
 
+// Fold the partial records: each thread folds a contiguous chunk into runs of
+// equal segment; a thread left with a single run merges it with the other
+// single-run lanes of its wave (shuffles, one atomic per segment per wave),
+// a thread with several runs applies its runs directly.
 template <int K>
-__global__ void k_parts_reduce(SegTargets<K> t, size_t stride, const BlockPart* parts, int nparts) {
+__global__ __launch_bounds__(1024) void k_parts_reduce(SegTargets<K> t, size_t stride, const BlockPart* parts,
+                                                       int nparts) {
     const int nt = blockDim.x * gridDim.x;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = (nparts + nt - 1) / nt;
-    const int beg = tid * per, end = min(nparts, beg + per);
+    const int beg = min(nparts, tid * per), end = min(nparts, beg + per);
     int cur = -1;
     double acc[K];
-    for (int q = beg; q <= end; q++) {
-        const int s = (q < end) ? parts[q].seg : -2;
-        if (s == -1) continue;
+    for (int k = 0; k < K; k++) acc[k] = 0.0;
+    for (int q = beg; q < end; q++) {
+        const int s = parts[q].seg;
+        if (s < 0) continue;
         if (s != cur) {
             if (cur >= 0)
                 for (int k = 0; k < K; k++) red_apply(t.base[k] + (size_t)cur * stride, acc[k], t.is_max[k], t.ordered[k]);
             cur = s;
-            if (s >= 0)
-                for (int k = 0; k < K; k++) acc[k] = parts[q].v[k];
+            for (int k = 0; k < K; k++) acc[k] = parts[q].v[k];
         } else {
             for (int k = 0; k < K; k++) acc[k] = t.is_max[k] ? fmax(acc[k], parts[q].v[k]) : fmin(acc[k], parts[q].v[k]);
         }
+    }
+    // the last run of every thread goes through the wave-level merge
+    const bool valid = cur >= 0;
+    const int lane = lane_id();
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int lseg = __shfl(cur, leader);
+        const bool mine = valid && cur == lseg && ((pending >> lane) & 1ull);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double x = mine ? acc[k] : (t.is_max[k] ? -__builtin_huge_val() : __builtin_huge_val());
+            x = t.is_max[k] ? wave_max_d(x) : wave_min_d(x);
+            if (lane == leader) red_apply(t.base[k] + (size_t)lseg * stride, x, t.is_max[k], t.ordered[k]);
+        }
+        pending &= ~__ballot(mine);
     }
 }
 

@@ -145,6 +145,7 @@ class Engine : public EngineBase {
         nb_sort_ = (int)blocks_for(n_, kRsTile);
         for (int k = 0; k < 2; k++) { keys_[k] = dalloc<uint32_t>(n_); vals_[k] = dalloc<uint32_t>(n_); }
         hist_ = dalloc<uint32_t>((size_t)256 * nb_sort_);
+        digit_totals_ = dalloc<uint32_t>(256);
         d_total_cells_ = dalloc<uint32_t>(1);
         // per-segment tables
         std::vector<MatT<R>> hm(mats_.size());
@@ -341,7 +342,7 @@ class Engine : public EngineBase {
     }
     template <int K>
     void fold_parts(int phase, const SegTargets<K>& t, unsigned nblocks) {
-        launch(phase, "parts_reduce", k_parts_reduce<K>, dim3(1), dim3(256), t, sizeof(SegState) / 8,
+        launch(phase, "parts_reduce", k_parts_reduce<K>, dim3(1), dim3(1024), t, sizeof(SegState) / 8,
                (const BlockPart*)w_.parts, (int)nblocks);
     }
 
@@ -363,10 +364,10 @@ class Engine : public EngineBase {
             const int shift = 8 * p;
             launch(PH_SORT, "rs_hist", k_rs_hist, dim3(nb_sort_), dim3(kRsThreads), (const uint32_t*)keys_[ka], n_,
                    shift, hist_, nb_sort_);
-            launch(PH_SORT, "rs_scan", k_rs_scan, dim3(1), dim3(1024), hist_, 256 * nb_sort_);
+            launch(PH_SORT, "rs_scan", k_rs_scan, dim3(256), dim3(256), hist_, nb_sort_, digit_totals_);
             launch(PH_SORT, "rs_scatter", k_rs_scatter, dim3(nb_sort_), dim3(kRsThreads), (const uint32_t*)keys_[ka],
                    (const uint32_t*)vals_[ka], keys_[1 - ka], vals_[1 - ka], n_, shift, (const uint32_t*)hist_,
-                   nb_sort_);
+                   nb_sort_, (const uint32_t*)digit_totals_);
             ka = 1 - ka;
         }
         const auto th = hmax_target(0);
@@ -882,6 +883,7 @@ class Engine : public EngineBase {
     uint32_t* keys_[2] = {nullptr, nullptr};
     uint32_t* vals_[2] = {nullptr, nullptr};
     uint32_t* hist_ = nullptr;
+    uint32_t* digit_totals_ = nullptr;
     uint32_t* d_total_cells_ = nullptr;
     int nb_sort_ = 0, passes_ = 0, nparts_max_ = 0;
     uint32_t capacity_ = 0;

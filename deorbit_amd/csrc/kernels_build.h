@@ -192,38 +192,54 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* keys, in
     hist[(size_t)t * nb + blockIdx.x] = h[t];
 }
 
-// Exclusive scan of m counters in place (one workgroup).
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* hist, int m) {
-    __shared__ uint32_t part[1024];
+// Exclusive scan of one digit's row of block counts (one workgroup per digit);
+// the row total goes to totals[digit].  The scatter adds the digit base.
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* hist, int nb, uint32_t* totals) {
+    __shared__ uint32_t part[256];
     const int t = threadIdx.x;
-    const int per = (m + 1023) / 1024;
-    const int beg = t * per, end = min(m, beg + per);
+    uint32_t* row = hist + (size_t)blockIdx.x * nb;
+    const int per = (nb + 255) / 256;
+    const int beg = t * per, end = min(nb, beg + per);
     uint32_t s = 0;
-    for (int i = beg; i < end; i++) s += hist[i];
+    for (int i = beg; i < end; i++) s += row[i];
     part[t] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (t >= off) ? part[t - off] : 0u;
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
     uint32_t run = part[t] - s;
     for (int i = beg; i < end; i++) {
-        uint32_t v = hist[i];
-        hist[i] = run;
+        const uint32_t v = row[i];
+        row[i] = run;
         run += v;
     }
+    if (t == 255) totals[blockIdx.x] = part[255];
 }
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin,
                                                           uint32_t* kout, uint32_t* vout, int64_t n,
-                                                          int shift, const uint32_t* hist, int nb) {
+                                                          int shift, const uint32_t* hist, int nb,
+                                                          const uint32_t* totals) {
     __shared__ uint32_t base[256];
     __shared__ uint32_t wc[4][256];
     __shared__ uint32_t pre[4][256];
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
-    base[t] = hist[(size_t)t * nb + blockIdx.x];
+    // digit base = exclusive scan of the digit totals (each block redoes this 256-entry scan)
+    const uint32_t tot = totals[t];
+    base[t] = tot;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = (t >= off) ? base[t - off] : 0u;
+        __syncthreads();
+        base[t] += v;
+        __syncthreads();
+    }
+    const uint32_t dbase = base[t] - tot;
+    __syncthreads();
+    base[t] = dbase + hist[(size_t)t * nb + blockIdx.x];
     for (int q = 0; q < 4; q++) wc[q][t] = 0;
     __syncthreads();
     const int64_t tile = (int64_t)blockIdx.x * kRsTile;

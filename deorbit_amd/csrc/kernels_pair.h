@@ -35,6 +35,28 @@ __device__ __forceinline__ R cell_gap(int d, R o, R cw) {
     return g > R(0) ? g : R(0);
 }
 
+// Fast reciprocal / square root: hardware approximations in FP32 (~1 ulp),
+// IEEE operations in FP64.
+__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double f_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double f_sqrt(double x) { return sqrt(x); }
+
+// Branch-free M4 shape and slope on q = r/h (q >= 0):
+//   f(q)  = t^3/4 - s^3          (1 - 1.5q^2 + 0.75q^3 for q < 1, (2-q)^3/4 for 1 <= q < 2)
+//   f'(q) = 3 s^2 - 0.75 t^2     (-3q + 2.25q^2 for q < 1, -0.75(2-q)^2 for 1 <= q < 2)
+// with t = max(2 - q, 0), s = max(1 - q, 0); W = f/(pi h^3), dW/dr = f'/(pi h^4).
+template <class R>
+__device__ __forceinline__ R m4_shape(R q) {
+    const R t = smax(R(2) - q, R(0)), s = smax(R(1) - q, R(0));
+    return R(0.25) * t * t * t - s * s * s;
+}
+template <class R>
+__device__ __forceinline__ R m4_slope(R q) {
+    const R t = smax(R(2) - q, R(0)), s = smax(R(1) - q, R(0));
+    return R(3) * s * s - R(0.75) * t * t;
+}
+
 // Would the reference's own 27-cell stencil (sph.hpp:163-182) reach particle
 // j from particle i in this density round?  Needed only when 2h_i can exceed
 // the reference's cell width; reproduces its neighbour set exactly.
@@ -163,6 +185,15 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
                 ob[1] = oracle_bin(g, p, 1, c.y[i]);
                 ob[2] = oracle_bin(g, p, 2, c.z[i]);
             }
+            // contribution of one candidate; FP32 tests r^2 (W is continuous at 2h),
+            // FP64 tests r < 2h exactly as the reference does
+            auto term = [&](const V4& q, R bx, R by, R bz) -> R {
+                const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+                const R r2 = rx * rx + ry * ry + rz * rz;
+                const R rr = f_sqrt(r2);
+                const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
+                return in ? q.w * (sig * m4_shape(rr * inv_h)) : R(0);
+            };
             R s = R(0);
             for (int dx = -rad; dx <= rad; dx++) {
                 const int nx = cx + dx;
@@ -188,15 +219,18 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
                         } else {
                             bx = me.x; by = me.y; bz = me.z;
                         }
-                        for (uint32_t k = rg.x; k < rg.y; k++) {
-                            const V4 q = w.dens_rec[k];
-                            const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
-                            const R r2 = rx * rx + ry * ry + rz * rz;
-                            if (r2 < th2) {
-                                if (emu && !oracle_reaches(g, p, ob, c.x[k], c.y[k], c.z[k])) continue;
-                                const R rr = d_sqrt(r2);
-                                if (sizeof(R) == 8 && !(rr < two_h)) continue;
-                                s += q.w * (sig * kern_shape(rr * inv_h));
+                        if (!emu) {
+                            uint32_t k = rg.x;
+                            for (; k + 1 < rg.y; k += 2) {       // two loads in flight
+                                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1];
+                                s += term(q0, bx, by, bz);
+                                s += term(q1, bx, by, bz);
+                            }
+                            if (k < rg.y) s += term(w.dens_rec[k], bx, by, bz);
+                        } else {
+                            for (uint32_t k = rg.x; k < rg.y; k++) {
+                                if (!oracle_reaches(g, p, ob, c.x[k], c.y[k], c.z[k])) continue;
+                                s += term(w.dens_rec[k], bx, by, bz);
                             }
                         }
                     }
@@ -232,13 +266,6 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
     own_offset<R>(c, g, i, w.cellc[i], w.dens_rec[i], ox, oy, oz);
     eos_epilogue<R>(c, w, i, w.par[seg], info_mat(inf), w.rho[i], c.h[i], ox, oy, oz);
 }
-
-// Fast reciprocal / square root: hardware approximations in FP32 (~1 ulp),
-// IEEE operations in FP64.
-__device__ __forceinline__ float f_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double f_rcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float f_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double f_sqrt(double x) { return sqrt(x); }
 
 // Fused strain rate + forces (sph.hpp:212-258 and 277-317; the Metal
 // strain_forces kernel, sph_force.metal:308-365, fuses the same two loops).
@@ -329,9 +356,7 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                             const R ihb = f_rcp(hbar);
                             const R qq = r * ihb;
                             const R sgm = ihb * ihb * ihb * R(1.0 / kPi);
-                            R shp;
-                            if (qq < R(1)) shp = qq * (R(-3) + R(2.25) * qq);
-                            else { const R t2 = smax(R(2) - qq, R(0)); shp = R(-0.75) * t2 * t2; }
+                            const R shp = m4_slope(qq);
                             const R fr = sgm * ihb * shp * f_rcp(r);    // dW/dr / r
                             const R gxw = rx * fr, gyw = ry * fr, gzw = rz * fr;
                             const V4 vj = w.frec_vel[k];
@@ -358,7 +383,7 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                             du += R(0.5) * mj * term * vg;
                             R mterm = term;
                             if (eps_as > R(0)) {
-                                R f = kern_shape(qq) * inv_fdp;
+                                R f = m4_shape(qq) * inv_fdp;
                                 f = f * f;
                                 f = f * f;
                                 mterm += (rart_i + tj.w) * f;
