@@ -51,5 +51,24 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+CLI_SRC = os.path.join(HERE, "cli", "dsph_gpu_ic.cpp")
+CLI = os.path.join(HERE, "bin", "dsph_gpu_ic")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The gpu_ic-compatible driver: host C++ over the C ABI, linked to libdsph.so."""
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", CLI_SRC, "-o", CLI, f"-L{HERE}", "-ldsph",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building dsph_gpu_ic")
+    return CLI
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_cli(verbose=True))

@@ -183,6 +183,59 @@ def ensemble_instance(member: dict, steps: int = 500):
                   name=f"ensemble{member['index']:03d}", steps=steps)
 
 
+def _arange(start, stop, step):
+    # float stepping exactly as boundary_daemon.py:113-117 does it (accumulated sums)
+    out, x = [], start
+    while (x < stop) if step > 0 else (x > stop):
+        out.append(x)
+        x += step
+    return np.array(out)
+
+
+def grazing_ic(theta_deg: float, v: float, cppr: int = 8, D: float = 1000.0, L=None, tgt_mat: int = 0,
+               imp_mat: int = 1, rho_t: float = 2700.0, rho_imp: float = 7800.0):
+    """The bounce-off campaign's IC (boundary_daemon.py:119-158): a half-space target
+    (x from -0.4 Lx to 0.6 Lx, y centred, z below 0) with a 6 dx frozen wall on the
+    sides and bottom, then an iron sphere of radius D/2 released 3 dx above the
+    surface at -0.35 Lx, moving at angle theta.  Rows are written target first
+    (x, then y, then z) and sphere last (i, j, k), as the daemon writes them.
+    ``L = (Lx, Ly, Lz)`` overrides the default max(20 km, 30D) x max(4 km, 6D) x
+    max(6 km, 8D) box (used to make small test cases)."""
+    a = D / 2.0
+    dx = a / cppr
+    th = math.radians(theta_deg)
+    vx, vz = v * math.cos(th), -v * math.sin(th)
+    Lx, Ly, Lz = L if L is not None else (max(20e3, 30 * D), max(4e3, 6 * D), max(6e3, 8 * D))
+    wall = 6.0 * dx
+    xs = _arange(-0.4 * Lx + dx / 2, 0.6 * Lx, dx)
+    ys = _arange(-Ly / 2 + dx / 2, Ly / 2, dx)
+    zs = _arange(-dx / 2, -Lz, -dx)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+    fx = (np.abs(X - xs[0]) < wall) | (np.abs(X - xs[-1]) < wall)
+    fxy = fx | (np.abs(Y - ys[0]) < wall) | (np.abs(Y - ys[-1]) < wall)
+    fixed_t = (fxy | (Z < zs[-1] + wall)).astype(np.uint8)
+    z0 = a + 3 * dx
+    x0 = -0.35 * Lx
+    na = int(math.ceil(a / dx)) + 1
+    r = np.arange(-na, na + 1)
+    I, J, K = np.meshgrid(r, r, r, indexing="ij")
+    sel = (I * dx) ** 2 + (J * dx) ** 2 + (K * dx) ** 2 <= a * a
+    I, J, K = I[sel], J[sel], K[sel]
+    pos = np.concatenate([np.stack([X, Y, Z], 1), np.stack([I * dx + x0, J * dx, K * dx + z0], 1)])
+    n_t, n_i = len(X), len(I)
+    vel = np.zeros_like(pos)
+    vel[n_t:, 0] = vx
+    vel[n_t:, 2] = vz
+    mass = np.concatenate([np.full(n_t, rho_t * dx ** 3), np.full(n_i, rho_imp * dx ** 3)])
+    mat = np.concatenate([np.full(n_t, tgt_mat, np.int32), np.full(n_i, imp_mat, np.int32)])
+    fixed = np.concatenate([fixed_t, np.zeros(n_i, np.uint8)])
+    lo = (xs[0] - dx, ys[0] - dx, zs[-1] - dx)
+    hi = (xs[-1] + dx, ys[-1] + dx, z0 + a + 0.5 * Lz)
+    return dict(pos=pos, vel=vel, mass=mass, u=np.zeros(len(mass)), mat=mat, fixed=fixed, dx=dx, lo=lo, hi=hi,
+                meta=dict(name="grazing", theta=theta_deg, v=v, n_target=n_t, n_impactor=n_i))
+
+
 def tiny_impact(R=4, nx=16, ny=10, nz=6, theta=30.0, speed=6000.0, strength=True, frac=True,
                 seed=7, jitter=0.01):
     """A few-thousand-particle impact with the full physics, for fast parity tests."""

@@ -122,3 +122,18 @@ def test_checkpoint_roundtrip_plain_and_extended(tmp_path):
     for k in ("hh", "acc", "dudt", "dSdt"):
         assert np.array_equal(back[k], st[k])
     assert not os.path.exists(p + ".tmp")
+
+
+def test_grazing_ic_reproduces_the_campaign_geometry(tmp_path):
+    # results/bounce_off_phase_diagram.json records N = 5,900,349 for every campaign run
+    ic = ics.grazing_ic(2.0, 12000.0)
+    assert len(ic["mass"]) == 5900349 and ic["dx"] == 62.5
+    assert ic["meta"]["n_impactor"] == 2109
+    imp = ic["mat"] == 1
+    assert np.all(ic["fixed"][imp] == 0) and ic["fixed"][~imp].sum() > 0
+    assert np.allclose(ic["vel"][imp][0], [12000 * np.cos(np.radians(2)), 0, -12000 * np.sin(np.radians(2))])
+    small = ics.grazing_ic(10.0, 8000.0, cppr=4, D=100.0, L=(1500.0, 400.0, 500.0))
+    p = os.path.join(tmp_path, "g.bin")
+    formats.write_ic(p, small)
+    back = formats.read_ic(p)
+    assert np.array_equal(back["fixed"], small["fixed"]) and np.array_equal(back["mat"], small["mat"])
