@@ -139,10 +139,16 @@ def main():
     kb = roofline.kernel_bytes("mixed")
     force_ms = prof["force"][0] / max(1, prof["force"][1])
     achieved = kb["force"] * n / (force_ms * 1e-3) / 1e9
+    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile of this
+    # kernel and workload, used only when it was recorded for these device sources
+    rec = roofline.recorded_traffic(os.path.join(ROOT, "profiles"),
+                                    "k_force<float>", f"config{args.config}")
     roof = {"bound": "hbm", "kernel": "k_force<float>", "achieved": round(achieved, 2),
             "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
-            "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5), "traffic": None,
-            "bytes_per_particle": kb["force"], "mean_launch_ms": round(force_ms, 4),
+            "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5),
+            "traffic": None if rec is None else rec["bytes"],
+            "bytes_per_particle": kb["force"], "algorithmic_bytes": kb["force"] * n,
+            "mean_launch_ms": round(force_ms, 4), "source_digest": roofline.source_digest(),
             "step_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5)}
     diag = s.diagnostics()
     s.close()

@@ -45,10 +45,11 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)std::max
 
 enum Phase {
     PH_STEP_BEGIN, PH_KICK_DRIFT, PH_GRID, PH_KEYS, PH_SORT, PH_PERMUTE, PH_ROUND, PH_DENSITY,
-    PH_EOS, PH_FORCE, PH_FINISH, PH_DIAG, PH_IO, PH_COUNT
+    PH_EOS, PH_FORCE, PH_FINISH, PH_DIAG, PH_IO, PH_REDUCE, PH_COUNT
 };
 static const char* kPhaseName[PH_COUNT] = {"step_begin", "kick_drift", "grid", "keys", "sort", "permute",
-                                           "round", "density", "eos", "force", "finish", "diag", "io"};
+                                           "round", "density", "eos", "force", "finish", "diag", "io",
+                                           "reduce"};
 
 struct HostParticles {
     std::vector<double> pos, vel, mass, u, h, D, eps_act, S;   // S: n x 6 IC-order rows
@@ -340,9 +341,11 @@ class Engine : public EngineBase {
         t.ordered[0] = 0;
         return t;
     }
+    // Fold the per-block partials of the kernel just launched (its own phase
+    // keeps only that kernel's time; the fold is accounted as "reduce").
     template <int K>
-    void fold_parts(int phase, const SegTargets<K>& t, unsigned nblocks) {
-        launch(phase, "parts_reduce", k_parts_reduce<K>, dim3(1), dim3(1024), t, sizeof(SegState) / 8,
+    void fold_parts(int, const SegTargets<K>& t, unsigned nblocks) {
+        launch(PH_REDUCE, "parts_reduce", k_parts_reduce<K>, dim3(1), dim3(1024), t, sizeof(SegState) / 8,
                (const BlockPart*)w_.parts, (int)nblocks);
     }
 

@@ -51,3 +51,35 @@ def step_bytes(precision: str = "mixed", passes: int = 3, iterations: int = 2) -
     k = kernel_bytes(precision)
     return (k["kick_drift"] + k["keys"] + passes * k["sort_pass"] + k["permute"]
             + (iterations - 1) * k["density"] + k["density_last"] + k["force"] + k["finish"])
+
+
+def source_digest() -> str:
+    """Short digest of the device sources; ties a recorded counter profile to a build."""
+    import hashlib
+    import os
+    csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".h", ".hip")):
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def recorded_traffic(folder: str, kernel: str, workload: str) -> dict | None:
+    """HBM bytes per launch recorded by ``rocpd_summary traffic`` (a ``traffic:``
+    line in any summary under `folder`) for this kernel, workload and source
+    digest; None when absent or recorded for other sources."""
+    import glob
+    import os
+    digest = source_digest()
+    paths = sorted(glob.glob(os.path.join(folder, "*.md")) + glob.glob(os.path.join(folder, "*.txt")))
+    for path in paths:
+        with open(path) as f:
+            lines = [ln for ln in f if ln.startswith("traffic:")]
+        for line in lines:
+            kv = dict(tok.split("=", 1) for tok in line[len("traffic:"):].split())
+            if kv.get("kernel") == kernel and kv.get("workload") == workload and kv.get("digest") == digest:
+                return {"bytes": float(kv["bytes"]), "fetch_factor": float(kv["fetch_factor"]),
+                        "write_factor": float(kv["write_factor"])}
+    return None
