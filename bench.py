@@ -53,9 +53,12 @@ def timed_run(sysm, steps, graph=True):
     return time.perf_counter() - t0
 
 
-def gpu_leg(ic, precision, warmup, steps):
+def gpu_leg(ic, precision, warmup, steps, autotune=True):
     s = dsph.make_system(ic, precision)
     s.init()
+    if autotune:
+        chosen = s.autotune(3)
+        log(f"autotune ({'fp64' if precision == dsph.FP64 else 'mixed'}): {chosen}")
     s.run(warmup, 0.0, True)
     el = timed_run(s, steps, True)
     return s, el
@@ -106,6 +109,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the library's default tuning instead of the autotuner's choice")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -128,7 +133,7 @@ def main():
     n = len(ic["mass"])
     log(f"workload {ic['meta']['name']}: n={n}")
 
-    s, el = gpu_leg(ic, dsph.MIXED, args.warmup, args.steps)
+    s, el = gpu_leg(ic, dsph.MIXED, args.warmup, args.steps, not args.no_autotune)
     value = n * args.steps / el
     ms = 1e3 * el / args.steps
     log(f"mixed: {ms:.3f} ms/step  {value / 1e6:.1f} M particle-steps/s")
@@ -137,13 +142,16 @@ def main():
         if cnt:
             log(f"  {k:11s} {tot / args.profile_steps:8.3f} ms/step  ({cnt} launches)")
     kb = roofline.kernel_bytes("mixed")
-    force_ms = prof["force"][0] / max(1, prof["force"][1])
+    # dominant kernel: the force pass, over neighbour lists or by cell walks
+    # (whichever the tuning selected)
+    phase = "force_list" if prof.get("force_list", (0.0, 0))[0] > prof["force"][0] else "force"
+    kname = "k_force_list<float>" if phase == "force_list" else "k_force<float>"
+    force_ms = prof[phase][0] / max(1, prof[phase][1])
     achieved = kb["force"] * n / (force_ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile of this
     # kernel and workload, used only when it was recorded for these device sources
-    rec = roofline.recorded_traffic(os.path.join(ROOT, "profiles"),
-                                    "k_force<float>", f"config{args.config}")
-    roof = {"bound": "hbm", "kernel": "k_force<float>", "achieved": round(achieved, 2),
+    rec = roofline.recorded_traffic(os.path.join(ROOT, "profiles"), kname, f"config{args.config}")
+    roof = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
             "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5),
             "traffic": None if rec is None else rec["bytes"],
@@ -155,7 +163,7 @@ def main():
 
     fp64 = None
     if not args.no_fp64:
-        s64, el64 = gpu_leg(ic, dsph.FP64, min(args.warmup, 5), max(5, args.steps // 4))
+        s64, el64 = gpu_leg(ic, dsph.FP64, min(args.warmup, 5), max(5, args.steps // 4), not args.no_autotune)
         v64 = n * max(5, args.steps // 4) / el64
         s64.close()
         fp64 = {"value": v64, "ms_per_step": 1e3 * el64 / max(5, args.steps // 4),

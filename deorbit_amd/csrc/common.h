@@ -53,6 +53,10 @@ struct SegState {
     int32_t rounds;
     int32_t bad;                           // non-zero: capacity or numeric trouble
     double hmax_final;
+    uint32_t list_bad;                     // non-zero: some h outgrew the neighbour-list skin
+    uint32_t pad_;
+    unsigned long long list_walk_steps;    // force passes that walked cells for that reason
+    unsigned long long list_overflow;      // particle lists that did not fit (cumulative)
 };
 
 // Device-side carried particle state, one buffer set.
@@ -89,6 +93,18 @@ struct Work {
     uint32_t* cellc;                 // packed (cx | cy << 10 | cz << 20) of the sort cell
     uint2* table;                    // per sort cell: [start, end) in the current order
     float* cell_hmax;                // per sort cell: upper bound of h over its particles
+    // Neighbour lists, built by the first density iteration of a step and read
+    // by the second iteration and the force pass.  Entries come in quads of
+    // four indices; quad q of particle i sits at
+    // nbr[((i >> 6) * (nbr_cap / 4) + q) * 64 + (i & 63)], so one wave reading
+    // its q-th quads touches 1 KiB of contiguous memory.  nbr_cnt[i] =
+    // kNbrOverflow marks a list that did not fit (that particle walks the cells).
+    uint4* nbr;
+    uint32_t* nbr_cnt;
+    R* h0;                           // h at the start of the build (list radius reference)
+    int32_t nbr_cap;                 // entries per particle (a multiple of 4); 0 = lists off
+    R skin_build;                    // list radius factor (1 + skin)
+    R skin_check;                    // lists stay valid while h <= skin_check * h0
     BlockPart* parts;                // per-block partial reductions
     const MatT<R>* mats;
     const SegPar* par;

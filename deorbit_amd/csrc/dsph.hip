@@ -45,11 +45,12 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)std::max
 
 enum Phase {
     PH_STEP_BEGIN, PH_KICK_DRIFT, PH_GRID, PH_KEYS, PH_SORT, PH_PERMUTE, PH_ROUND, PH_DENSITY,
-    PH_EOS, PH_FORCE, PH_FINISH, PH_DIAG, PH_IO, PH_REDUCE, PH_COUNT
+    PH_EOS, PH_FORCE, PH_FINISH, PH_DIAG, PH_IO, PH_REDUCE, PH_DENSITY_BUILD, PH_DENSITY_LIST, PH_FORCE_LIST,
+    PH_COUNT
 };
 static const char* kPhaseName[PH_COUNT] = {"step_begin", "kick_drift", "grid", "keys", "sort", "permute",
                                            "round", "density", "eos", "force", "finish", "diag", "io",
-                                           "reduce"};
+                                           "reduce", "density_build", "density_list", "force_list"};
 
 struct HostParticles {
     std::vector<double> pos, vel, mass, u, h, D, eps_act, S;   // S: n x 6 IC-order rows
@@ -80,6 +81,8 @@ class Engine : public EngineBase {
         tune_.cell_factor = 1.5;
         tune_.xcd_remap = 1;
         tune_.cell_capacity_per_particle = 4;
+        tune_.neighbor_capacity = 0;
+        tune_.list_skin = 0.06;
         for (auto& e : prof_total_) e = 0.0;
         for (auto& e : prof_count_) e = 0;
     }
@@ -142,6 +145,8 @@ class Engine : public EngineBase {
         nparts_max_ = (int)blocks_for(n_, 64) + 1;
         w.parts = dalloc<BlockPart>(nparts_max_);
         alloc_table();
+        w.h0 = dalloc<R>(n_);
+        alloc_lists();
         // sort buffers
         nb_sort_ = (int)blocks_for(n_, kRsTile);
         for (int k = 0; k < 2; k++) { keys_[k] = dalloc<uint32_t>(n_); vals_[k] = dalloc<uint32_t>(n_); }
@@ -212,6 +217,33 @@ class Engine : public EngineBase {
         records_ = false;
         have_dt_ = false;
         h_host_set_ = true;
+    }
+
+    // Neighbour lists: nbr_cap entries per particle, in 64-particle blocks.
+    // The automatic capacity is about 1.5x the mean list length of a uniform
+    // lattice at the default skin.
+    // Automatic choice (neighbor_capacity 0): lists for the FP64 path, whose
+    // pair math is costly enough that visiting only listed pairs pays for the
+    // build; cell walks for the FP32 path, where the gathers of a list cost
+    // more than the walk's rejected candidates.
+    int list_capacity() const {
+        if (tune_.neighbor_capacity < 0) return 0;
+        if (tune_.neighbor_capacity > 0) return tune_.neighbor_capacity;
+        return sizeof(R) == 8 ? 128 : 0;
+    }
+    void alloc_lists() {
+        const int cap = (list_capacity() + 3) / 4 * 4;
+        const int64_t padded = (n_ + 63) / 64 * 64;
+        w_.nbr_cap = cap;
+        w_.nbr = cap ? dalloc<uint4>((size_t)padded * (size_t)(cap / 4)) : nullptr;
+        w_.nbr_cnt = dalloc<uint32_t>(n_);
+        w_.skin_build = (R)(1.0 + tune_.list_skin);
+        w_.skin_check = (R)(1.0 + 0.5 * tune_.list_skin);
+        lists_built_ = false;
+    }
+    void free_ptr(void* old) {
+        auto it = std::find(allocs_.begin(), allocs_.end(), old);
+        if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
     }
 
     void alloc_table() {
@@ -380,6 +412,7 @@ class Engine : public EngineBase {
         cur_ = 1 - cur_;
         built_ = true;
         records_ = false;
+        lists_built_ = false;
         if (carry_all) {
             // Outside a step every per-particle field is state the caller may read
             // or feed back (System keeps them between calls), so carry them through
@@ -398,16 +431,32 @@ class Engine : public EngineBase {
         }
     }
 
+    // Up to four rounds of two iterations; rounds a segment does not need exit
+    // at once on the device.  Round 0's first iteration builds the neighbour
+    // lists that its second iteration and the force pass read.
     void density() {
         const int bq = tune_.block_pair;
         const unsigned nb = blocks_for(n_, bq);
+        const int lists = w_.nbr_cap > 0 ? 1 : 0;
         for (int r = 0; r < 4; r++) {
             launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, r);
             const auto th = hmax_target(r + 1);
-            launch(PH_DENSITY, "density", k_density<R, false>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap, th);
-            launch(PH_DENSITY, "density", k_density<R, true>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap, th);
+            if (r == 0 && lists)
+                launch(PH_DENSITY_BUILD, "density_build", k_density_build<R>, dim3(nb), dim3(bq), cur(), w_,
+                       tune_.xcd_remap);
+            else
+                launch(PH_DENSITY, "density", k_density<R, false>, dim3(nb), dim3(bq), cur(), w_, r,
+                       tune_.xcd_remap, lists, th);
+            launch(PH_DENSITY, "density", k_density<R, true>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap,
+                   lists, th);
             fold_parts(PH_DENSITY, th, nb);
+            if (r == 0 && lists) {
+                launch(PH_DENSITY_LIST, "density_list", k_density_list<R>, dim3(nb), dim3(bq), cur(), w_,
+                       tune_.xcd_remap, th);
+                fold_parts(PH_DENSITY, th, nb);
+            }
         }
+        lists_built_ = lists != 0;
         launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, 4);
         records_ = true;
     }
@@ -420,7 +469,13 @@ class Engine : public EngineBase {
         const int bq = tune_.block_pair;
         const auto td = dt_target();
         launch(PH_ROUND, "reset_dt", k_reset_dt, dim3(1), dim3(256), d_st_, nseg_);
-        launch(PH_FORCE, "force", k_force<R>, dim3(blocks_for(n_, bq)), dim3(bq), cur(), w_, tune_.xcd_remap, td);
+        const unsigned nb = blocks_for(n_, bq);
+        const int lists = lists_built_ ? 1 : 0;
+        launch(PH_FORCE, "force", k_force<R>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, lists, td);
+        if (lists) {
+            fold_parts(PH_FORCE, td, nb);
+            launch(PH_FORCE_LIST, "force_list", k_force_list<R>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, td);
+        }
         fold_parts(PH_FORCE, td, blocks_for(n_, bq));
         have_dt_ = true;
     }
@@ -643,6 +698,7 @@ class Engine : public EngineBase {
         sync();
         if (f == DSPH_F_POS) built_ = false;
         records_ = false;
+        lists_built_ = false;      // host-written state: the force pass walks cells
     }
 
     // ---------------------------------------------------------------- diagnostics
@@ -704,6 +760,8 @@ class Engine : public EngineBase {
         out->dt_next = have_dt_ ? bits_to_double(s.dt_next) : 0.0;
         out->steps = s.steps;
         out->density_rounds = s.rounds;
+        out->list_walk_steps = (int64_t)s.list_walk_steps;
+        out->list_overflow = (int64_t)s.list_overflow;
     }
 
     // ---------------------------------------------------------------- profile / tuning
@@ -737,6 +795,16 @@ class Engine : public EngineBase {
             nt.cell_factor = t.cell_factor;
         }
         nt.xcd_remap = t.xcd_remap ? 1 : 0;
+        if (t.list_skin != 0.0) {
+            if (!(t.list_skin > 0.0 && t.list_skin <= 1.0)) throw Err{DSPH_E_ARG, "list_skin must be in (0, 1]"};
+            nt.list_skin = t.list_skin;
+        }
+        if (t.neighbor_capacity) {
+            if (t.neighbor_capacity > 0 && (t.neighbor_capacity < 8 || t.neighbor_capacity > 4096))
+                throw Err{DSPH_E_ARG, "neighbor_capacity must be in [8, 4096], or negative for no lists"};
+            nt.neighbor_capacity = t.neighbor_capacity > 0 ? t.neighbor_capacity : -1;
+        }
+        const bool lists_changed = nt.neighbor_capacity != tune_.neighbor_capacity || nt.list_skin != tune_.list_skin;
         if (t.cell_capacity_per_particle && t.cell_capacity_per_particle != tune_.cell_capacity_per_particle) {
             if (t.cell_capacity_per_particle < 1 || t.cell_capacity_per_particle > 64)
                 throw Err{DSPH_E_ARG, "cell_capacity_per_particle must be in [1, 64]"};
@@ -745,16 +813,19 @@ class Engine : public EngineBase {
                 tune_ = nt;
                 // the table is reallocated; everything else is kept
                 release_graphs();
-                for (void* old : {(void*)w_.table, (void*)w_.cell_hmax}) {
-                    auto it = std::find(allocs_.begin(), allocs_.end(), old);
-                    if (it != allocs_.end()) { (void)hipFree(*it); allocs_.erase(it); }
-                }
+                for (void* old : {(void*)w_.table, (void*)w_.cell_hmax}) free_ptr(old);
                 alloc_table();
                 built_ = false;
                 records_ = false;
             }
         }
         tune_ = nt;
+        if (lists_changed && n_ > 0) {
+            release_graphs();
+            if (w_.nbr) free_ptr(w_.nbr);
+            free_ptr(w_.nbr_cnt);
+            alloc_lists();
+        }
         release_graphs();
     }
 
@@ -796,7 +867,8 @@ class Engine : public EngineBase {
         dsph_tuning best = tune_;
         double best_ms = 1e300;
         const int blocks[3] = {64, 128, 256};
-        const double cells[3] = {1.0, 1.5, 2.0};
+        const double cells[4] = {1.0, 1.25, 1.5, 2.0};
+        const int caps[2] = {-1, 128};          // cell walks, neighbour lists
         hipEvent_t e0, e1;
         HIPC(hipEventCreate(&e0));
         HIPC(hipEventCreate(&e1));
@@ -809,13 +881,22 @@ class Engine : public EngineBase {
             built_ = false;
             records_ = false;
         };
+        auto set_lists = [&](int cap) {
+            if (cap == tune_.neighbor_capacity) return;
+            tune_.neighbor_capacity = cap;
+            if (w_.nbr) free_ptr(w_.nbr);
+            free_ptr(w_.nbr_cnt);
+            alloc_lists();
+        };
+        for (int cap : caps)
         for (double cf : cells) {
             for (int bq : blocks) {
+                release_graphs();
+                set_lists(cap);
                 dsph_tuning t = tune_;
                 t.block_pair = bq;
                 t.cell_factor = cf;
                 tune_ = t;
-                release_graphs();
                 restore();
                 step(0, 0.0);           // warm-up (also rebuilds for this grid)
                 HIPC(hipEventRecord(e0, stream_));
@@ -829,8 +910,9 @@ class Engine : public EngineBase {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        tune_ = best;
         release_graphs();
+        set_lists(best.neighbor_capacity);
+        tune_ = best;
         restore();
         sync();
         for (void* q : saved) (void)hipFree(q);
@@ -904,6 +986,7 @@ class Engine : public EngineBase {
     int nchunks_ = 0;
     dsph_tuning tune_{};
     bool built_ = false, records_ = false, have_dt_ = false;
+    bool lists_built_ = false;     // the last density pass built neighbour lists
     bool profiling_ = false, capturing_ = false;
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> ev_pool_;

@@ -12,6 +12,14 @@
 namespace dsph {
 
 constexpr int kMaxStencil = 16;               // cap on the per-particle stencil radius in cells
+constexpr uint32_t kNbrOverflow = 0xffffffffu; // neighbour count of a list that did not fit
+
+// Positive float rounded up to the next float at or above the double d.
+__device__ __forceinline__ float float_at_least(double d) {
+    float f = (float)d;
+    if ((double)f < d) f = __uint_as_float(__float_as_uint(f) + 1u);
+    return f;
+}
 
 // ---------------------------------------------------------------- step control
 // mode 0: each segment takes its device CFL step; 1: host scalar dt; 2: per-segment host dt.
@@ -46,6 +54,7 @@ __global__ void k_build_begin(SegState* st, int nseg) {
         for (int r = 0; r < 4; r++) S.active[r] = 0;
         S.rounds = 0;
         S.hmax_final = 0.0;
+        S.list_bad = 0u;
     }
 }
 
@@ -313,7 +322,11 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
         if (key != prev) w.table[key].x = (uint32_t)i;
         if (key != next) w.table[key].y = (uint32_t)(i + 1);
         const SegGrid& g = w.grid[seg];
+        w.h0[i] = h;
         if (key != g.cell_base + g.ncells - 1u) {
+            // per-cell bound of h at the start of the step (list build radius;
+            // the density epilogue raises it to the final h)
+            atomicMax((unsigned int*)&w.cell_hmax[key], __float_as_uint(float_at_least((double)h)));
             const int cx = cell_coord(g, 0, x), cy = cell_coord(g, 1, y), cz = cell_coord(g, 2, z);
             w.cellc[i] = cellc_pack(cx, cy, cz);
             V4 rec;
@@ -371,6 +384,7 @@ __global__ void k_round(SegState* st, SegGrid* grid, const SegPar* par, int nseg
                 g.ocw_min = cmin;
             }
         } else {
+            if (S.list_bad) S.list_walk_steps++;
             if (!S.active[0]) {
                 S.rounds = 0;
                 S.hmax_final = bits_to_double(S.hmax[0]);

@@ -143,14 +143,177 @@ __device__ __forceinline__ void eos_epilogue(const Carried<R>& c, const Work<R>&
     w.frec_sb[i] = sb;
 }
 
-// One density iteration of round r (sph.hpp:189-201):
+// ---------------------------------------------------------------- neighbour lists
+// Slot of quad q (entries 4q..4q+3) of particle i's list (see Work::nbr).
+__device__ __forceinline__ size_t nbr_quad(int64_t i, uint32_t q, int cap) {
+    return ((size_t)(i >> 6) * (size_t)(cap >> 2) + (size_t)q) * 64u + (size_t)(i & 63);
+}
+__device__ __forceinline__ uint32_t quad_get(const uint4& v, int s) {
+    return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
+}
+
+// Position of particle i relative to the frame of neighbour k's sort cell:
+// FP32 path, own-cell offset shifted by the cell delta (so the pair separation
+// is formed from numbers below one cell width); FP64 path, absolute position.
+template <class R>
+__device__ __forceinline__ void rel_base(const Work<R>& w, uint32_t k, int cx, int cy, int cz, R cw, R ox, R oy,
+                                         R oz, R ax, R ay, R az, R& bx, R& by, R& bz) {
+    if (sizeof(R) == 4) {
+        const uint32_t ck = w.cellc[k];
+        const int dx = (int)(ck & 1023u) - cx, dy = (int)((ck >> 10) & 1023u) - cy, dz = (int)(ck >> 20) - cz;
+        bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
+    } else {
+        bx = ax; by = ay; bz = az;
+    }
+}
+
+// ---------------------------------------------------------------- density
+// Per-particle constants of one density iteration.
+template <class R>
+struct DensLane {
+    using V4 = typename Vec4<R>::T;
+    V4 me;                 // own density record
+    R h, ox, oy, oz, two_h, th2, inv_h, sig;
+    int cx, cy, cz;
+    bool emu;              // 2h may exceed the reference's cell width this round
+    int ob[3];             // the reference's cell of i (emulation only)
+
+    __device__ void setup(const Carried<R>& c, const SegPar& p, const SegGrid& g, int64_t i, uint32_t cc) {
+        cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
+        two_h = R(2) * h;
+        th2 = two_h * two_h;
+        inv_h = R(1) / h;
+        sig = inv_h * inv_h * inv_h * R(1.0 / kPi);
+        emu = (double)two_h > 0.999 * g.ocw_min;
+        ob[0] = ob[1] = ob[2] = 0;
+        if (emu) {
+            ob[0] = oracle_bin(g, p, 0, c.x[i]);
+            ob[1] = oracle_bin(g, p, 1, c.y[i]);
+            ob[2] = oracle_bin(g, p, 2, c.z[i]);
+        }
+    }
+    // contribution of one candidate; FP32 tests r^2 (W is continuous at 2h),
+    // FP64 tests r < 2h exactly as the reference does
+    __device__ __forceinline__ R term(const V4& q, R bx, R by, R bz) const {
+        const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+        const R r2 = rx * rx + ry * ry + rz * rz;
+        const R rr = f_sqrt(r2);
+        const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
+        return in ? q.w * (sig * m4_shape(rr * inv_h)) : R(0);
+    }
+    // frame of the sort cell at stencil offset (dx, dy, dz)
+    __device__ __forceinline__ void frame(int dx, int dy, int dz, R cw, R& bx, R& by, R& bz) const {
+        if (sizeof(R) == 4) {
+            bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
+        } else {
+            bx = me.x; by = me.y; bz = me.z;
+        }
+    }
+};
+
+// Visit the sort cells whose nearest point is closer than `reach` to particle
+// i, in lexicographic cell order; fn(key, dx, dy, dz, gap^2) per cell.
+template <class R, class F>
+__device__ __forceinline__ void for_cells(const SegGrid& g, int cx, int cy, int cz, R ox, R oy, R oz, R reach,
+                                          F&& fn) {
+    const R cw = (R)g.cw;
+    const R reach2 = reach * reach;
+    int rad = (int)ceil((double)reach * g.inv_cw);
+    rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
+    for (int dx = -rad; dx <= rad; dx++) {
+        const int nx = cx + dx;
+        if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
+        const R gx = cell_gap(dx, ox, cw);
+        const R gx2 = gx * gx;
+        if (gx2 >= reach2) continue;
+        for (int dy = -rad; dy <= rad; dy++) {
+            const int ny = cy + dy;
+            if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
+            const R gy = cell_gap(dy, oy, cw);
+            const R gxy = gx2 + gy * gy;
+            if (gxy >= reach2) continue;
+            for (int dz = -rad; dz <= rad; dz++) {
+                const int nz = cz + dz;
+                if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
+                const R gz = cell_gap(dz, oz, cw);
+                const R gg = gxy + gz * gz;
+                if (gg >= reach2) continue;
+                fn(cell_key(g, nx, ny, nz), dx, dy, dz, gg);
+            }
+        }
+    }
+}
+
+// Sum over the sort cells within 2h of particle i, in lexicographic cell order
+// and cell-table order inside a cell.
+template <class R>
+__device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p, const SegGrid& g,
+                          const DensLane<R>& L) {
+    using V4 = typename Vec4<R>::T;
+    const R cw = (R)g.cw;
+    R s = R(0);
+    for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.two_h, [&](uint32_t key, int dx, int dy, int dz, R) {
+        const uint2 rg = w.table[key];
+        R bx, by, bz;
+        L.frame(dx, dy, dz, cw, bx, by, bz);
+        if (!L.emu) {
+            uint32_t k = rg.x;
+            for (; k + 1 < rg.y; k += 2) {       // two loads in flight
+                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1];
+                s += L.term(q0, bx, by, bz);
+                s += L.term(q1, bx, by, bz);
+            }
+            if (k < rg.y) s += L.term(w.dens_rec[k], bx, by, bz);
+        } else {
+            for (uint32_t k = rg.x; k < rg.y; k++) {
+                if (!oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
+                s += L.term(w.dens_rec[k], bx, by, bz);
+            }
+        }
+    });
+    return s;
+}
+
+// The new h of particle i from its density (sph.hpp:195-198).
+template <class R>
+__device__ __forceinline__ R density_h(const Carried<R>& c, const SegPar& p, int64_t i, R rho) {
+    const R h0 = (R)p.h_init;
+    const R mi = (R)c.mass[i];
+    return smin(R(2) * h0, smax(R(0.5) * h0, (R)p.eta * d_cbrt(mi / rho)));
+}
+
+// Does particle i read its neighbour list in the second density iteration of
+// round 0?  Only while its h has not outgrown the list skin.
+template <class R>
+__device__ __forceinline__ bool dens_list_usable(const Work<R>& w, int64_t i, R h) {
+    return w.nbr_cnt[i] != kNbrOverflow && h <= w.skin_check * w.h0[i];
+}
+
+// Store the new density and h; the last iteration of a round also flags h
+// growth beyond the list skin and runs the EOS epilogue.
+template <class R, bool LAST>
+__device__ __forceinline__ R density_store(const Carried<R>& c, const Work<R>& w, int64_t i, int seg,
+                                           const SegPar& p, R rho, bool finite, int lists, R ox, R oy, R oz) {
+    const R hn = density_h<R>(c, p, i, rho);
+    w.rho[i] = rho;
+    c.h[i] = hn;
+    // lists built this step stay usable by the force pass only while every
+    // final h of the segment is within the skin of its build-time value
+    if (LAST && lists && finite && !(hn <= w.skin_check * w.h0[i])) atomicOr(&w.st[seg].list_bad, 1u);
+    if (LAST) eos_epilogue<R>(c, w, i, p, info_mat(c.info[i]), rho, hn, ox, oy, oz);
+    return hn;
+}
+
+// One density iteration of round r by walking the cells (sph.hpp:189-201):
 //   rho_i = max(sum_j m_j W(r_ij, h_i) over r_ij < 2 h_i, 1e-30)
 //   h_i   = clamp(eta (m_i / rho_i)^(1/3), h_init / 2, 2 h_init)
 // LAST marks the second iteration of the round: it also reduces max h for the
-// round test and runs the EOS epilogue.
+// round test and runs the EOS epilogue.  In the second iteration of round 0
+// with lists built, lanes whose list is usable are left to k_density_list;
+// the others walk and mark their list unusable for the force pass too.
 template <class R, bool LAST>
-__global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r, int remap, SegTargets<1> tg) {
-    using V4 = typename Vec4<R>::T;
+__global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r, int remap, int lists,
+                                                 SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
     int seg = 0;
@@ -159,99 +322,157 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
         seg = info_seg(c.info[i]);
         act = w.st[seg].active[r] != 0;
     }
+    const bool by_list = LAST && lists && r == 0;
     double hnew_d = 0.0;
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
-        const R h = c.h[i];
-        const uint32_t cc = w.cellc[i];
-        const V4 me = w.dens_rec[i];
-        R ox, oy, oz;
-        own_offset<R>(c, g, i, cc, me, ox, oy, oz);
-        const bool finite = isfinite((double)ox) && isfinite((double)oy) && isfinite((double)oz);
-        R rho;
-        if (finite) {
-            const int cx = cc & 1023u, cy = (cc >> 10) & 1023u, cz = cc >> 20;
-            const R cw = (R)g.cw;
-            const R two_h = R(2) * h, th2 = two_h * two_h;
-            int rad = (int)ceil((double)two_h * g.inv_cw);
-            rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
-            const R inv_h = R(1) / h;
-            const R sig = inv_h * inv_h * inv_h * R(1.0 / kPi);
-            const bool emu = (double)two_h > 0.999 * g.ocw_min;
-            int ob[3] = {0, 0, 0};
-            if (emu) {
-                ob[0] = oracle_bin(g, p, 0, c.x[i]);
-                ob[1] = oracle_bin(g, p, 1, c.y[i]);
-                ob[2] = oracle_bin(g, p, 2, c.z[i]);
-            }
-            // contribution of one candidate; FP32 tests r^2 (W is continuous at 2h),
-            // FP64 tests r < 2h exactly as the reference does
-            auto term = [&](const V4& q, R bx, R by, R bz) -> R {
-                const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
-                const R r2 = rx * rx + ry * ry + rz * rz;
-                const R rr = f_sqrt(r2);
-                const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
-                return in ? q.w * (sig * m4_shape(rr * inv_h)) : R(0);
-            };
-            R s = R(0);
-            for (int dx = -rad; dx <= rad; dx++) {
-                const int nx = cx + dx;
-                if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
-                const R gx = cell_gap(dx, ox, cw);
-                const R gx2 = gx * gx;
-                if (gx2 >= th2) continue;
-                for (int dy = -rad; dy <= rad; dy++) {
-                    const int ny = cy + dy;
-                    if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
-                    const R gy = cell_gap(dy, oy, cw);
-                    const R gxy = gx2 + gy * gy;
-                    if (gxy >= th2) continue;
-                    for (int dz = -rad; dz <= rad; dz++) {
-                        const int nz = cz + dz;
-                        if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
-                        const R gz = cell_gap(dz, oz, cw);
-                        if (gxy + gz * gz >= th2) continue;
-                        const uint2 rg = w.table[cell_key(g, nx, ny, nz)];
-                        R bx, by, bz;
-                        if (sizeof(R) == 4) {
-                            bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
-                        } else {
-                            bx = me.x; by = me.y; bz = me.z;
-                        }
-                        if (!emu) {
-                            uint32_t k = rg.x;
-                            for (; k + 1 < rg.y; k += 2) {       // two loads in flight
-                                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1];
-                                s += term(q0, bx, by, bz);
-                                s += term(q1, bx, by, bz);
-                            }
-                            if (k < rg.y) s += term(w.dens_rec[k], bx, by, bz);
-                        } else {
-                            for (uint32_t k = rg.x; k < rg.y; k++) {
-                                if (!oracle_reaches(g, p, ob, c.x[k], c.y[k], c.z[k])) continue;
-                                s += term(w.dens_rec[k], bx, by, bz);
-                            }
-                        }
-                    }
-                }
-            }
-            rho = smax(s, rho_floor<R>());
+        DensLane<R> L;
+        L.h = c.h[i];
+        if (by_list && dens_list_usable<R>(w, i, L.h)) {
+            act = false;                           // k_density_list owns this particle
         } else {
-            rho = rho_floor<R>();
+            if (by_list) w.nbr_cnt[i] = kNbrOverflow;
+            const uint32_t cc = w.cellc[i];
+            L.me = w.dens_rec[i];
+            own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
+            const bool finite = isfinite((double)L.ox) && isfinite((double)L.oy) && isfinite((double)L.oz);
+            R rho = rho_floor<R>();
+            if (finite) {
+                L.setup(c, p, g, i, cc);
+                rho = smax(density_walk<R>(c, w, p, g, L), rho_floor<R>());
+            }
+            hnew_d = (double)density_store<R, LAST>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
         }
-        const R h0 = (R)p.h_init;
-        const R mi = (R)c.mass[i];
-        const R hn = smin(R(2) * h0, smax(R(0.5) * h0, (R)p.eta * d_cbrt(mi / rho)));
-        w.rho[i] = rho;
-        c.h[i] = hn;
-        hnew_d = (double)hn;
-        if (LAST) eos_epilogue<R>(c, w, i, p, info_mat(c.info[i]), rho, hn, ox, oy, oz);
     }
     if (LAST) {
         const double v[1] = {hnew_d};
         block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
     }
+}
+
+// Second density iteration of round 0 for the particles whose list is usable:
+// the list holds the walk's pairs in the walk's order, so the sum is the one
+// the walk would give.  Four entries are read ahead so their loads overlap.
+template <class R>
+__global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, int remap, SegTargets<1> tg) {
+    using V4 = typename Vec4<R>::T;
+    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    int seg = 0;
+    bool act = false;
+    if (i < w.n) {
+        seg = info_seg(c.info[i]);
+        act = w.st[seg].active[0] != 0 && dens_list_usable<R>(w, i, c.h[i]);
+    }
+    double hnew_d = 0.0;
+    if (act) {
+        const SegPar& p = w.par[seg];
+        const SegGrid& g = w.grid[seg];
+        DensLane<R> L;
+        L.h = c.h[i];
+        const uint32_t cc = w.cellc[i];
+        L.me = w.dens_rec[i];
+        own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
+        L.setup(c, p, g, i, cc);
+        const R cw = (R)g.cw;
+        const uint32_t nc = w.nbr_cnt[i];
+        const int cap = w.nbr_cap;
+        R s = R(0);
+        auto add = [&](uint32_t k, const V4& q) {
+            if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) return;
+            R bx, by, bz;
+            rel_base<R>(w, k, L.cx, L.cy, L.cz, cw, L.ox, L.oy, L.oz, L.me.x, L.me.y, L.me.z, bx, by, bz);
+            s += L.term(q, bx, by, bz);
+        };
+        for (uint32_t e = 0; e < nc; e += 4) {
+            const uint4 kk = w.nbr[nbr_quad(i, e >> 2, cap)];
+            const V4 q0 = w.dens_rec[kk.x], q1 = w.dens_rec[kk.y], q2 = w.dens_rec[kk.z], q3 = w.dens_rec[kk.w];
+            add(kk.x, q0);
+            if (e + 1 < nc) add(kk.y, q1);
+            if (e + 2 < nc) add(kk.z, q2);
+            if (e + 3 < nc) add(kk.w, q3);
+        }
+        const R rho = smax(s, rho_floor<R>());
+        hnew_d = (double)density_store<R, true>(c, w, i, seg, p, rho, true, 1, L.ox, L.oy, L.oz);
+    }
+    const double v[1] = {hnew_d};
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
+}
+
+// First density iteration of round 0, which also writes the neighbour lists:
+// while walking, every candidate j with
+//   r_ij < (1+skin)(h_i + max(h_i, hcell_j))
+// enters i's list in walk order (hcell_j: bound of h over j's sort cell at
+// the start of the step).  That covers r < 2h_i and r < h_i + h_j for as long
+// as no h grows by more than the skin.  Lists that do not fit are marked.
+template <class R>
+__global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, int remap) {
+    using V4 = typename Vec4<R>::T;
+    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    if (i >= w.n) return;
+    const int seg = info_seg(c.info[i]);
+    uint32_t cnt = kNbrOverflow;
+    if (w.st[seg].active[0]) {
+        const SegPar& p = w.par[seg];
+        const SegGrid& g = w.grid[seg];
+        DensLane<R> L;
+        L.h = c.h[i];
+        const uint32_t cc = w.cellc[i];
+        L.me = w.dens_rec[i];
+        own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
+        const bool finite = isfinite((double)L.ox) && isfinite((double)L.oy) && isfinite((double)L.oz);
+        R rho = rho_floor<R>();
+        if (finite) {
+            L.setup(c, p, g, i, cc);
+            const R cw = (R)g.cw;
+            const R sk = w.skin_build;
+            const R hm = (R)bits_to_double(w.st[seg].hmax[0]);
+            const int cap = w.nbr_cap;
+            R s = R(0);
+            cnt = 0u;
+            // entries are gathered four at a time and stored as one 16-byte quad;
+            // unused lanes of a last partial quad keep valid indices
+            uint4 quad = make_uint4((uint32_t)i, (uint32_t)i, (uint32_t)i, (uint32_t)i);
+            for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, sk * (L.h + smax(L.h, hm)),
+                         [&](uint32_t key, int dx, int dy, int dz, R gg) {
+                const R rb = sk * (L.h + smax(L.h, (R)w.cell_hmax[key]));
+                const R rb2 = rb * rb;
+                if (gg >= rb2) return;
+                const uint2 rg = w.table[key];
+                R bx, by, bz;
+                L.frame(dx, dy, dz, cw, bx, by, bz);
+                for (uint32_t k = rg.x; k < rg.y; k++) {
+                    const V4 q = w.dens_rec[k];
+                    const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+                    const R r2 = rx * rx + ry * ry + rz * rz;
+                    if (r2 < rb2) {
+                        if (cnt < (uint32_t)cap) {
+                            const uint32_t sl = cnt & 3u;
+                            quad.x = sl == 0u ? k : quad.x;
+                            quad.y = sl == 1u ? k : quad.y;
+                            quad.z = sl == 2u ? k : quad.z;
+                            quad.w = sl == 3u ? k : quad.w;
+                            if (sl == 3u) w.nbr[nbr_quad(i, cnt >> 2, cap)] = quad;
+                        }
+                        cnt++;
+                    }
+                    if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
+                    s += L.term(q, bx, by, bz);
+                }
+            });
+            if (cnt > (uint32_t)cap) {
+                cnt = kNbrOverflow;
+                atomicAdd(&w.st[seg].list_overflow, 1ull);
+            } else if (cnt & 3u) {
+                w.nbr[nbr_quad(i, cnt >> 2, cap)] = quad;
+            }
+            rho = smax(s, rho_floor<R>());
+        }
+        density_store<R, false>(c, w, i, seg, p, rho, finite, 0, L.ox, L.oy, L.oz);
+    }
+    w.nbr_cnt[i] = cnt;
 }
 
 // EOS epilogue on its own (System::compute_forces entry without a density pass).
@@ -267,6 +488,7 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
     eos_epilogue<R>(c, w, i, w.par[seg], info_mat(inf), w.rho[i], c.h[i], ox, oy, oz);
 }
 
+// ---------------------------------------------------------------- forces
 // Fused strain rate + forces (sph.hpp:212-258 and 277-317; the Metal
 // strain_forces kernel, sph_force.metal:308-365, fuses the same two loops).
 // For each pair j != i with 0 < r < 2 hbar, hbar = (h_i + h_j)/2:
@@ -277,136 +499,95 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
 //   a     += m_j (S_i d_i/rho_i^2 + S_j d_j/rho_j^2) . grad W
 // then the Jaumann stress rate, the deviatoric work and the CFL step.
 template <class R>
-__global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, SegTargets<1> tg) {
+struct ForceLane {
     using V4 = typename Vec4<R>::T;
-    const int64_t n = w.n;
-    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
-    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
-    const bool in = i < n;
-    int seg = 0;
-    double dt_i = 0.0;
-    bool dt_valid = false;
-    if (in) {
-        const uint32_t inf = c.info[i];
-        seg = info_seg(inf);
-        const int mat = info_mat(inf);
-        const SegPar& p = w.par[seg];
-        const SegGrid& g = w.grid[seg];
-        const V4 pi = w.frec_pos[i];
-        const V4 vi = w.frec_vel[i];
+    V4 pi, vi, sai, sbi;
+    R hi, rho_i, pio, csig_i, rart_i;
+    R alpha, beta, eps, eps_as, alpha_u, inv_fdp, ui, Pi;
+    R ox, oy, oz, cw;
+    int cx, cy, cz;
+    R L00 = 0, L01 = 0, L02 = 0, L10 = 0, L11 = 0, L12 = 0, L20 = 0, L21 = 0, L22 = 0;
+    R ax = 0, ay = 0, az = 0, du = 0;
+
+    __device__ void setup(const Carried<R>& c, const Work<R>& w, const SegPar& p, const SegGrid& g, int64_t i,
+                          uint32_t cc) {
+        pi = w.frec_pos[i];
+        vi = w.frec_vel[i];
         const V4 ti = w.frec_th[i];
-        const V4 sai = w.frec_sa[i];
-        const V4 sbi = w.frec_sb[i];
-        const uint32_t cc = w.cellc[i];
-        R ox, oy, oz;
-        own_offset<R>(c, g, i, cc, w.dens_rec[i], ox, oy, oz);
-        const bool finite = isfinite((double)ox) && isfinite((double)oy) && isfinite((double)oz);
-        const R hi = pi.w, rho_i = ti.x, pio = ti.y, csig_i = ti.z, rart_i = ti.w;
-        const R alpha = (R)p.alpha, beta = (R)p.beta, eps = (R)p.eps, eps_as = (R)p.eps_as;
-        const R alpha_u = (R)p.alpha_u;
-        const R inv_fdp = (eps_as > R(0)) ? R(1) / kern_shape(R(1) / (R)p.eta) : R(0);
-        R L00 = 0, L01 = 0, L02 = 0, L10 = 0, L11 = 0, L12 = 0, L20 = 0, L21 = 0, L22 = 0;
-        R ax = 0, ay = 0, az = 0, du = 0;
-        if (finite) {
-            const int cx = cc & 1023u, cy = (cc >> 10) & 1023u, cz = cc >> 20;
-            const R cw = (R)g.cw;
-            const R rad_r = hi + (R)w.st[seg].hmax_final;
-            const R rad2 = rad_r * rad_r;
-            int rad = (int)ceil((double)rad_r * g.inv_cw);
-            rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
-            const R ui = (R)c.u[i];
-            const R Pi = w.P[i];
-            for (int dx = -rad; dx <= rad; dx++) {
-                const int nx = cx + dx;
-                if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
-                const R gx0 = cell_gap(dx, ox, cw);
-                const R gx2 = gx0 * gx0;
-                if (gx2 >= rad2) continue;
-                for (int dy = -rad; dy <= rad; dy++) {
-                    const int ny = cy + dy;
-                    if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
-                    const R gy0 = cell_gap(dy, oy, cw);
-                    const R gxy = gx2 + gy0 * gy0;
-                    if (gxy >= rad2) continue;
-                    for (int dz = -rad; dz <= rad; dz++) {
-                        const int nz = cz + dz;
-                        if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
-                        const R gz0 = cell_gap(dz, oz, cw);
-                        const R gg = gxy + gz0 * gz0;
-                        if (gg >= rad2) continue;
-                        const uint32_t key = cell_key(g, nx, ny, nz);
-                        const R reach = hi + (R)w.cell_hmax[key];
-                        if (gg >= reach * reach) continue;
-                        const uint2 rg = w.table[key];
-                        R bx, by, bz;
-                        if (sizeof(R) == 4) {
-                            bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
-                        } else {
-                            bx = pi.x; by = pi.y; bz = pi.z;
-                        }
-                        for (uint32_t k = rg.x; k < rg.y; k++) {
-                            const V4 q = w.frec_pos[k];
-                            const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
-                            const R r2 = rx * rx + ry * ry + rz * rz;
-                            const R th = hi + q.w;                  // 2 hbar
-                            if (!(r2 < th * th) || r2 == R(0)) continue;
-                            const R r = f_sqrt(r2);
-                            if (sizeof(R) == 8 && !(r < th)) continue;
-                            const R hbar = R(0.5) * th;
-                            const R ihb = f_rcp(hbar);
-                            const R qq = r * ihb;
-                            const R sgm = ihb * ihb * ihb * R(1.0 / kPi);
-                            const R shp = m4_slope(qq);
-                            const R fr = sgm * ihb * shp * f_rcp(r);    // dW/dr / r
-                            const R gxw = rx * fr, gyw = ry * fr, gzw = rz * fr;
-                            const V4 vj = w.frec_vel[k];
-                            const V4 tj = w.frec_th[k];
-                            const V4 saj = w.frec_sa[k];
-                            const V4 sbj = w.frec_sb[k];
-                            const R dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-                            const R Vj = sbj.z;
-                            const R t0 = Vj * dvx, t1 = Vj * dvy, t2v = Vj * dvz;
-                            L00 += t0 * gxw; L01 += t0 * gyw; L02 += t0 * gzw;
-                            L10 += t1 * gxw; L11 += t1 * gyw; L12 += t1 * gzw;
-                            L20 += t2v * gxw; L21 += t2v * gyw; L22 += t2v * gzw;
-                            const R dvr = -(dvx * rx + dvy * ry + dvz * rz);   // v_ij . r_ij
-                            R pij = R(0);
-                            if (dvr < R(0)) {
-                                const R mu = hbar * dvr * f_rcp(r2 + eps * hbar * hbar);
-                                const R cbar = R(0.5) * (csig_i + tj.z);
-                                const R rbar = R(0.5) * (rho_i + tj.x);
-                                pij = (-alpha * cbar * mu + beta * mu * mu) * f_rcp(rbar);
-                            }
-                            const R term = pio + tj.y + pij;
-                            const R vg = -(dvx * gxw + dvy * gyw + dvz * gzw);  // v_ij . grad W
-                            const R mj = vj.w;
-                            du += R(0.5) * mj * term * vg;
-                            R mterm = term;
-                            if (eps_as > R(0)) {
-                                R f = m4_shape(qq) * inv_fdp;
-                                f = f * f;
-                                f = f * f;
-                                mterm += (rart_i + tj.w) * f;
-                            }
-                            const R mm = mj * mterm;
-                            const R sxx = sai.x + saj.x, syy = sai.y + saj.y, szz = sai.z + saj.z;
-                            const R sxy = sai.w + saj.w, sxz = sbi.x + sbj.x, syz = sbi.y + sbj.y;
-                            ax += (sxx * gxw + sxy * gyw + sxz * gzw) * mj - gxw * mm;
-                            ay += (sxy * gxw + syy * gyw + syz * gzw) * mj - gyw * mm;
-                            az += (sxz * gxw + syz * gyw + szz * gzw) * mj - gzw * mm;
-                            if (alpha_u > R(0)) {
-                                const R Pj = w.P[k];
-                                const R uj = (R)c.u[k];
-                                const R rbar = R(0.5) * (rho_i + tj.x);
-                                const R vsig = d_sqrt(d_fabs(Pi - Pj) / rbar);
-                                du += alpha_u * mj * vsig * (ui - uj) * (sgm * ihb * shp) / rbar;
-                            }
-                        }
-                    }
-                }
-            }
+        sai = w.frec_sa[i];
+        sbi = w.frec_sb[i];
+        hi = pi.w; rho_i = ti.x; pio = ti.y; csig_i = ti.z; rart_i = ti.w;
+        alpha = (R)p.alpha; beta = (R)p.beta; eps = (R)p.eps; eps_as = (R)p.eps_as; alpha_u = (R)p.alpha_u;
+        inv_fdp = (eps_as > R(0)) ? R(1) / kern_shape(R(1) / (R)p.eta) : R(0);
+        cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
+        cw = (R)g.cw;
+        ui = alpha_u > R(0) ? (R)c.u[i] : R(0);
+        Pi = alpha_u > R(0) ? w.P[i] : R(0);
+    }
+
+    // neighbour k at separation (rx, ry, rz), r2 = |r|^2 < th^2, th = 2 hbar
+    __device__ __forceinline__ void body(const Carried<R>& c, const Work<R>& w, uint32_t k, R rx, R ry, R rz, R r2,
+                                         R th, const V4& vj, const V4& tj, const V4& saj, const V4& sbj) {
+        const R r = f_sqrt(r2);
+        if (sizeof(R) == 8 && !(r < th)) return;
+        const R hbar = R(0.5) * th;
+        const R ihb = f_rcp(hbar);
+        const R qq = r * ihb;
+        const R sgm = ihb * ihb * ihb * R(1.0 / kPi);
+        const R shp = m4_slope(qq);
+        const R fr = sgm * ihb * shp * f_rcp(r);    // dW/dr / r
+        const R gxw = rx * fr, gyw = ry * fr, gzw = rz * fr;
+        const R dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+        const R Vj = sbj.z;
+        const R t0 = Vj * dvx, t1 = Vj * dvy, t2v = Vj * dvz;
+        L00 += t0 * gxw; L01 += t0 * gyw; L02 += t0 * gzw;
+        L10 += t1 * gxw; L11 += t1 * gyw; L12 += t1 * gzw;
+        L20 += t2v * gxw; L21 += t2v * gyw; L22 += t2v * gzw;
+        // Written without branches so that every record component is used in
+        // one basic block and each record is fetched by a single 16-byte load.
+        const R dvr = -(dvx * rx + dvy * ry + dvz * rz);   // v_ij . r_ij
+        // Monaghan viscosity on approaching pairs (mu = 0 otherwise gives Pi = 0)
+        const R mu = hbar * smin(dvr, R(0)) * f_rcp(r2 + eps * hbar * hbar);
+        const R cbar = R(0.5) * (csig_i + tj.z);
+        const R rbar = R(0.5) * (rho_i + tj.x);
+        const R pij = (-alpha * cbar * mu + beta * mu * mu) * f_rcp(rbar);
+        const R term = pio + tj.y + pij;
+        const R vg = -(dvx * gxw + dvy * gyw + dvz * gzw);  // v_ij . grad W
+        const R mj = vj.w;
+        du += R(0.5) * mj * term * vg;
+        // artificial stress; inv_fdp = 0 when it is off
+        R f = m4_shape(qq) * inv_fdp;
+        f = f * f;
+        f = f * f;
+        const R mterm = term + (rart_i + tj.w) * f;
+        const R mm = mj * mterm;
+        const R sxx = sai.x + saj.x, syy = sai.y + saj.y, szz = sai.z + saj.z;
+        const R sxy = sai.w + saj.w, sxz = sbi.x + sbj.x, syz = sbi.y + sbj.y;
+        ax += (sxx * gxw + sxy * gyw + sxz * gzw) * mj - gxw * mm;
+        ay += (sxy * gxw + syy * gyw + syz * gzw) * mj - gyw * mm;
+        az += (sxz * gxw + syz * gyw + szz * gzw) * mj - gzw * mm;
+        if (alpha_u > R(0)) {
+            const R Pj = w.P[k];
+            const R uj = (R)c.u[k];
+            const R rbar = R(0.5) * (rho_i + tj.x);
+            const R vsig = d_sqrt(d_fabs(Pi - Pj) / rbar);
+            du += alpha_u * mj * vsig * (ui - uj) * (sgm * ihb * shp) / rbar;
         }
-        // strain rate, spin, Jaumann rate and deviatoric power (sph.hpp:230-256)
+    }
+
+    // separation of candidate k seen from (bx, by, bz); true when in range
+    __device__ __forceinline__ bool in_range(const V4& q, R bx, R by, R bz, R& rx, R& ry, R& rz, R& r2,
+                                             R& th) const {
+        rx = bx - q.x; ry = by - q.y; rz = bz - q.z;
+        r2 = rx * rx + ry * ry + rz * rz;
+        th = hi + q.w;
+        return r2 < th * th && r2 != R(0);
+    }
+
+    // strain rate, spin, Jaumann rate and deviatoric power (sph.hpp:230-256),
+    // the outputs of particle i, and its CFL step (sph.hpp:350-355, in FP64)
+    __device__ double store(const Carried<R>& c, const Work<R>& w, int64_t i, int mat, const SegPar& p) {
+        const int64_t n = w.n;
         const R G = w.mats[mat].G;
         R epsw = R(0);
         R d6[6] = {0, 0, 0, 0, 0, 0};
@@ -443,12 +624,115 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
         w.eps_work[i] = epsw;
         w.ax[i] = ax; w.ay[i] = ay; w.az[i] = az;
         w.dudt[i] = du + epsw * w.dscale[i];
-        // CFL step (sph.hpp:350-355), evaluated in FP64
-        dt_i = p.cfl * (double)hi / ((double)csig_i * (1.0 + p.alpha) + 1e-30);
-        dt_valid = true;
+        return p.cfl * (double)hi / ((double)csig_i * (1.0 + p.alpha) + 1e-30);
+    }
+};
+
+// Does particle i read its neighbour list in the force pass?  Only when the
+// lists of this step were built, its own fit, and no h of its segment grew
+// beyond the skin (then the list holds every pair the walk would find, in the
+// walk's order).
+template <class R>
+__device__ __forceinline__ bool force_list_usable(const Work<R>& w, int64_t i, int seg, int lists) {
+    return lists && w.st[seg].list_bad == 0u && w.nbr_cnt[i] != kNbrOverflow;
+}
+
+// Force pass by walking the cells (radius h_i + max h, culled with the per-cell
+// bound of h).  With lists built, particles whose list is usable are left to
+// k_force_list.
+template <class R>
+__global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, int lists, SegTargets<1> tg) {
+    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    int seg = 0;
+    double dt_i = 0.0;
+    bool mine = false;
+    if (i < w.n) {
+        const uint32_t inf = c.info[i];
+        seg = info_seg(inf);
+        mine = !force_list_usable<R>(w, i, seg, lists);
+        if (mine) {
+            const SegPar& p = w.par[seg];
+            const SegGrid& g = w.grid[seg];
+            const uint32_t cc = w.cellc[i];
+            ForceLane<R> F;
+            F.setup(c, w, p, g, i, cc);
+            own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
+            if (isfinite((double)F.ox) && isfinite((double)F.oy) && isfinite((double)F.oz)) {
+                for_cells<R>(g, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.hi + (R)w.st[seg].hmax_final,
+                             [&](uint32_t key, int dx, int dy, int dz, R gg) {
+                    const R reach = F.hi + (R)w.cell_hmax[key];
+                    if (gg >= reach * reach) return;
+                    const uint2 rg = w.table[key];
+                    R bx, by, bz;
+                    if (sizeof(R) == 4) {
+                        bx = F.ox - (R)dx * F.cw; by = F.oy - (R)dy * F.cw; bz = F.oz - (R)dz * F.cw;
+                    } else {
+                        bx = F.pi.x; by = F.pi.y; bz = F.pi.z;
+                    }
+                    for (uint32_t k = rg.x; k < rg.y; k++) {
+                        R rx, ry, rz, r2, th;
+                        if (!F.in_range(w.frec_pos[k], bx, by, bz, rx, ry, rz, r2, th)) continue;
+                        F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
+                               w.frec_sb[k]);
+                    }
+                });
+            }
+            dt_i = F.store(c, w, i, info_mat(inf), p);
+        }
     }
     const double v[1] = {dt_i};
-    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, dt_valid);
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, mine);
+}
+
+// Force pass over the neighbour lists.  List entries are mostly pairs in
+// range, so every record of two entries is read up front and their loads
+// overlap.
+template <class R>
+__global__ __launch_bounds__(256) void k_force_list(Carried<R> c, Work<R> w, int remap, SegTargets<1> tg) {
+    using V4 = typename Vec4<R>::T;
+    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    int seg = 0;
+    double dt_i = 0.0;
+    bool mine = false;
+    if (i < w.n) {
+        const uint32_t inf = c.info[i];
+        seg = info_seg(inf);
+        mine = force_list_usable<R>(w, i, seg, 1);
+        if (mine) {
+            const SegPar& p = w.par[seg];
+            const SegGrid& g = w.grid[seg];
+            const uint32_t cc = w.cellc[i];
+            ForceLane<R> F;
+            F.setup(c, w, p, g, i, cc);
+            own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
+            const uint32_t nc = w.nbr_cnt[i];
+            const int cap = w.nbr_cap;
+            // two entries at a time, with all their records in flight together
+            auto two = [&](uint32_t k0, uint32_t k1, bool has1) {
+                const V4 q0 = w.frec_pos[k0], q1 = w.frec_pos[k1];
+                const V4 v0 = w.frec_vel[k0], v1 = w.frec_vel[k1];
+                const V4 t0 = w.frec_th[k0], t1 = w.frec_th[k1];
+                const V4 a0 = w.frec_sa[k0], a1 = w.frec_sa[k1];
+                const V4 b0 = w.frec_sb[k0], b1 = w.frec_sb[k1];
+                R bx, by, bz, rx, ry, rz, r2, th;
+                rel_base<R>(w, k0, F.cx, F.cy, F.cz, F.cw, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
+                if (F.in_range(q0, bx, by, bz, rx, ry, rz, r2, th)) F.body(c, w, k0, rx, ry, rz, r2, th, v0, t0, a0, b0);
+                rel_base<R>(w, k1, F.cx, F.cy, F.cz, F.cw, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
+                if (has1 && F.in_range(q1, bx, by, bz, rx, ry, rz, r2, th))
+                    F.body(c, w, k1, rx, ry, rz, r2, th, v1, t1, a1, b1);
+            };
+            for (uint32_t e = 0; e < nc; e += 4) {
+                const uint4 kk = w.nbr[nbr_quad(i, e >> 2, cap)];
+                two(kk.x, kk.y, e + 1 < nc);
+                if (e + 2 < nc) two(kk.z, kk.w, e + 3 < nc);
+            }
+            dt_i = F.store(c, w, i, info_mat(inf), p);
+        }
+    }
+    const double v[1] = {dt_i};
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, mine);
 }
 
 }  // namespace dsph

@@ -56,7 +56,8 @@ class Params(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block_pair", C.c_int32), ("block_point", C.c_int32), ("cell_factor", C.c_double),
-                ("xcd_remap", C.c_int32), ("cell_capacity_per_particle", C.c_int32)]
+                ("xcd_remap", C.c_int32), ("cell_capacity_per_particle", C.c_int32),
+                ("neighbor_capacity", C.c_int32), ("list_skin", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -65,7 +66,8 @@ class Diag(C.Structure):
                 ("vmax", C.c_double), ("umax", C.c_double), ("hmin", C.c_double),
                 ("hmax", C.c_double), ("rho_min", C.c_double), ("rho_max", C.c_double),
                 ("dmax", C.c_double), ("t", C.c_double), ("dt_next", C.c_double),
-                ("steps", C.c_int64), ("density_rounds", C.c_int32), ("nonfinite", C.c_int32)]
+                ("steps", C.c_int64), ("density_rounds", C.c_int32), ("nonfinite", C.c_int32),
+                ("list_walk_steps", C.c_int64), ("list_overflow", C.c_int64)]
 
 
 class Profile(C.Structure):

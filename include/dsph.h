@@ -72,6 +72,8 @@ typedef struct {
     double cell_factor;         /* sort-cell width = cell_factor * h_init        */
     int32_t xcd_remap;          /* 1 = map consecutive tiles onto one XCD        */
     int32_t cell_capacity_per_particle;  /* cell table size / particle count    */
+    int32_t neighbor_capacity;  /* neighbour-list entries per particle; 0 = auto, <0 = no lists */
+    double list_skin;           /* list radius = (1 + skin) x interaction radius */
 } dsph_tuning;
 
 /* Per-segment diagnostics, summed with compensated (Kahan/TwoSum) FP64. */
@@ -85,6 +87,8 @@ typedef struct {
     int64_t steps;
     int32_t density_rounds;     /* rounds run by the last density evaluation    */
     int32_t nonfinite;          /* count of non-finite pos/vel/u values         */
+    int64_t list_walk_steps;    /* force passes that walked cells: an h outgrew the list skin */
+    int64_t list_overflow;      /* particle neighbour lists that did not fit (cumulative)    */
 } dsph_diag;
 
 /* Per-phase device time from HIP events (profiling mode). */
@@ -161,9 +165,10 @@ int dsph_get_profile(dsph_ctx* ctx, dsph_profile* out);
 int dsph_reset_profile(dsph_ctx* ctx);
 int dsph_get_tuning(const dsph_ctx* ctx, dsph_tuning* out);
 int dsph_set_tuning(dsph_ctx* ctx, const dsph_tuning* t);
-/* Time `steps` steps for each candidate tuning (block sizes x cell widths)
- * on a scratch copy of the current state, keep the fastest, and restore the
- * state; the chosen tuning is returned and applied. */
+/* Time `steps` steps for each candidate tuning (neighbour lists on or off x
+ * cell widths x pair-kernel block sizes) on a scratch copy of the current
+ * state, keep the fastest, and restore the state; the chosen tuning is
+ * returned and applied. */
 int dsph_autotune(dsph_ctx* ctx, int32_t steps, dsph_tuning* chosen);
 
 #ifdef __cplusplus

@@ -51,6 +51,36 @@ def test_graph_replay_equals_stream_launches(precision, gpu):
     assert da["t"] == db["t"] and da["steps"] == db["steps"] == 7
 
 
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_neighbour_lists_give_the_cell_walk_results(precision, gpu):
+    # The lists hold the walk's pairs in the walk's order, so every way of
+    # falling back (lists off, lists overflowing, h outgrowing a tiny skin)
+    # must reproduce the same numbers: bit for bit in FP64; in FP32 the list
+    # and walk kernels may contract multiply-adds differently, so there the
+    # bar is FP32 round-off (scale-normalised 1e-6 after 12 steps).
+    ic = ics.config0()
+    steps = 12
+    ref, dref = run_states(ic, precision, steps, True, dict(neighbor_capacity=-1))
+    assert dref["list_walk_steps"] == 0 and dref["list_overflow"] == 0
+    n = len(ic["mass"])
+    runs = {}
+    on, don = run_states(ic, precision, steps, True, dict(neighbor_capacity=128))
+    assert don["list_overflow"] < 1e-3 * n * (steps + 1), don     # a few interface particles at most
+    runs["lists"] = on
+    small, dsmall = run_states(ic, precision, steps, True, dict(neighbor_capacity=40))
+    tight, dtight = run_states(ic, precision, steps, True, dict(neighbor_capacity=128, list_skin=1e-4))
+    assert dsmall["list_overflow"] > 0, dsmall
+    assert dtight["list_walk_steps"] > 0, dtight
+    runs["overflowing lists"] = small
+    runs["tiny skin"] = tight
+    for name, st in runs.items():
+        for f in FIELDS:
+            if precision == dsph.FP64:
+                assert np.array_equal(st[f], ref[f]), (name, f, scale_err(st[f], ref[f]))
+            else:
+                assert scale_err(st[f], ref[f]) < 1e-6, (name, f, scale_err(st[f], ref[f]))
+
+
 def test_tuning_changes_speed_not_results_beyond_roundoff(gpu):
     ic = ics.tiny_impact()
     a, _ = run_states(ic, dsph.FP64, 4, True)
