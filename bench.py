@@ -32,13 +32,16 @@ sys.path.insert(0, ROOT)
 
 from deorbit_amd import dsph, ics, roofline  # noqa: E402
 
-CONFIGS = {0: ics.config0, 1: ics.config1, 2: ics.config2, 4: ics.config4}
+CONFIGS = {0: ics.config0, 1: ics.config1, 2: ics.config2, 3: None, 4: ics.config4}
 WORKLOAD = {
     0: "config0: R=10dx iron sphere at 45 deg, 8 km/s, onto a 64x32x16 granite slab, no strength",
     1: "config1: R=40dx iron sphere at 3 deg, 8 km/s, into a 128x64x96 granite slab, strength+damage",
     2: "config2: R=60dx iron sphere at 2 deg, 8 km/s, into 192x96 columns of 32 water over 136 granite",
+    3: "config3: 256 independent iron-on-granite impacts in one batch (N logU[8k,200k], 1-45 deg, "
+       "4-12 km/s, impactor/target 0.05-0.3, strength+damage), 500 steps per instance",
     4: "config4: R=120dx iron sphere at 1.5 deg, 8 km/s, onto a 384x192x339 granite half-space",
 }
+ENSEMBLE_STEPS = 500       # steps per instance of the batched ensemble
 
 
 def log(*a):
@@ -54,7 +57,7 @@ def timed_run(sysm, steps, graph=True):
 
 
 def gpu_leg(ic, precision, warmup, steps, autotune=True):
-    s = dsph.make_system(ic, precision)
+    s = dsph.make_batch(ic, precision) if isinstance(ic, list) else dsph.make_system(ic, precision)
     s.init()
     if autotune:
         chosen = s.autotune(3)
@@ -129,9 +132,16 @@ def main():
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    ic = CONFIGS[args.config]()
-    n = len(ic["mass"])
-    log(f"workload {ic['meta']['name']}: n={n}")
+    if args.config == 3:
+        ic = ics.ensemble(256, ENSEMBLE_STEPS)
+        n = sum(len(m["mass"]) for m in ic)
+        log(f"workload ensemble of {len(ic)} impacts: n={n}")
+        cpu_ic = sorted(ic, key=lambda m: len(m["mass"]))[len(ic) // 2]     # the median instance
+    else:
+        ic = CONFIGS[args.config]()
+        n = len(ic["mass"])
+        log(f"workload {ic['meta']['name']}: n={n}")
+        cpu_ic = ics.impact_region(ic, 1_100_000)     # the CPU solver's share of a large impact
 
     s, el = gpu_leg(ic, dsph.MIXED, args.warmup, args.steps, not args.no_autotune)
     value = n * args.steps / el
@@ -172,7 +182,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(ic)
+        cpu = cpu_baseline(cpu_ic)
+        if cpu_ic is not ic:
+            what = "median-size ensemble instance" if args.config == 3 else "impact region"
+            cpu["sample"] = f"{what} ({len(cpu_ic['mass'])} particles): " + cpu["sample"]
         log(f"cpu: {cpu['value'] / 1e6:.3f} M particle-steps/s ({cpu['kind']}, {cpu['cores']} threads)")
 
     line = {
@@ -185,6 +198,11 @@ def main():
                    "graph": True, "t_sim_end": diag["t"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64": fp64,
     }
+    if args.config == 3:
+        # every batched step advances all 256 instances by one step
+        line["runs_per_hour"] = len(ic) * 3600.0 / (ENSEMBLE_STEPS * el / args.steps)
+        line["config"]["instances"] = len(ic)
+        line["config"]["steps_per_instance"] = ENSEMBLE_STEPS
     print(json.dumps(line), flush=True)
 
 

@@ -375,9 +375,12 @@ class Engine : public EngineBase {
     }
     // Fold the per-block partials of the kernel just launched (its own phase
     // keeps only that kernel's time; the fold is accounted as "reduce").
+    // About four records per thread, so the fold stays a few microseconds at
+    // any particle count (each thread's records are read one after another).
     template <int K>
     void fold_parts(int, const SegTargets<K>& t, unsigned nblocks) {
-        launch(PH_REDUCE, "parts_reduce", k_parts_reduce<K>, dim3(1), dim3(1024), t, sizeof(SegState) / 8,
+        const unsigned grid = std::max(1u, (nblocks + 4095u) / 4096u);
+        launch(PH_REDUCE, "parts_reduce", k_parts_reduce<K>, dim3(grid), dim3(1024), t, sizeof(SegState) / 8,
                (const BlockPart*)w_.parts, (int)nblocks);
     }
 
@@ -390,7 +393,7 @@ class Engine : public EngineBase {
         const auto tb = bbox_targets();
         launch(PH_KICK_DRIFT, "kick_drift", k_kick_drift<R>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_, move, tb);
         fold_parts(PH_KICK_DRIFT, tb, blocks_for(n_, bp));
-        launch(PH_GRID, "grid", k_grid, dim3(1), dim3(64), d_grid_, d_st_, (const SegPar*)d_par_, nseg_,
+        launch(PH_GRID, "grid", k_grid, dim3(1), dim3(kGridThreads), d_grid_, d_st_, (const SegPar*)d_par_, nseg_,
                tune_.cell_factor, capacity_, d_total_cells_);
         launch(PH_KEYS, "keys", k_keys<R>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_, keys_[0], vals_[0],
                (const uint32_t*)d_total_cells_);

@@ -94,46 +94,72 @@ __global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int
 // segment bbox, padded to whole 4x4x4 bricks, laid end to end in one key
 // space.  If the total exceeds the table capacity every width grows by 25 %
 // until it fits (coarser cells stay correct: stencils are sized in cells).
-__global__ void k_grid(SegGrid* grid, SegState* st, const SegPar* par, int nseg, double cell_factor,
-                       uint32_t capacity, uint32_t* total_out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One workgroup of kGridThreads threads: segments are handled a chunk at a
+// time, one per thread, and a block scan lays their key ranges end to end.
+constexpr int kGridThreads = 1024;
+__global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* st, const SegPar* par, int nseg,
+                                                       double cell_factor, uint32_t capacity, uint32_t* total_out) {
+    __shared__ unsigned long long s_scan[kGridThreads];
+    __shared__ unsigned long long s_carry;
+    const int t = threadIdx.x;
     double scale = 1.0;
     unsigned long long total = 0;
     for (int attempt = 0; attempt < 200; attempt++) {
-        total = 0;
-        for (int s = 0; s < nseg; s++) {
-            SegGrid& g = grid[s];
-            double lo[3], ext[3], emax = 0.0;
-            for (int a = 0; a < 3; a++) {
-                double l = ord_dec(st[s].bmin[a]), h = ord_dec(st[s].bmax[a]);
-                if (!(l <= h)) { l = 0.0; h = 0.0; }          // segment without finite particles
-                lo[a] = l;
-                ext[a] = h - l;
-                emax = ext[a] > emax ? ext[a] : emax;
+        if (t == 0) s_carry = 0;
+        __syncthreads();
+        for (int s0 = 0; s0 < nseg; s0 += kGridThreads) {
+            const int s = s0 + t;
+            unsigned long long nc = 0;
+            if (s < nseg) {
+                SegGrid& g = grid[s];
+                double lo[3], ext[3], emax = 0.0;
+                for (int a = 0; a < 3; a++) {
+                    double l = ord_dec(st[s].bmin[a]), h = ord_dec(st[s].bmax[a]);
+                    if (!(l <= h)) { l = 0.0; h = 0.0; }          // segment without finite particles
+                    lo[a] = l;
+                    ext[a] = h - l;
+                    emax = ext[a] > emax ? ext[a] : emax;
+                }
+                double cw = cell_factor * par[s].h_init * scale;
+                if (emax / cw > 1000.0) cw = emax / 1000.0;         // at most 1001 cells per axis
+                // 64 cells per brick plus one trailing "park" cell that collects
+                // particles with a non-finite position (never visited by a stencil)
+                nc = 64;
+                for (int a = 0; a < 3; a++) {
+                    g.lo[a] = lo[a];
+                    g.dim[a] = (int)floor(ext[a] / cw) + 1;
+                    g.bdim[a] = (g.dim[a] + 3) / 4;
+                    nc *= (unsigned long long)g.bdim[a];
+                }
+                nc += 1;
+                g.cw = cw;
+                g.inv_cw = 1.0 / cw;
+                g.ncells = (uint32_t)nc;
             }
-            double cw = cell_factor * par[s].h_init * scale;
-            if (emax / cw > 1000.0) cw = emax / 1000.0;         // at most 1001 cells per axis
-            // 64 cells per brick plus one trailing "park" cell that collects
-            // particles with a non-finite position (never visited by a stencil)
-            unsigned long long nc = 64;
-            for (int a = 0; a < 3; a++) {
-                g.lo[a] = lo[a];
-                g.dim[a] = (int)floor(ext[a] / cw) + 1;
-                g.bdim[a] = (g.dim[a] + 3) / 4;
-                nc *= (unsigned long long)g.bdim[a];
+            // inclusive scan of the chunk's cell counts
+            s_scan[t] = nc;
+            __syncthreads();
+            for (int o = 1; o < kGridThreads; o <<= 1) {
+                const unsigned long long add = t >= o ? s_scan[t - o] : 0ull;
+                __syncthreads();
+                s_scan[t] += add;
+                __syncthreads();
             }
-            nc += 1;
-            g.cw = cw;
-            g.inv_cw = 1.0 / cw;
-            g.cell_base = (uint32_t)total;
-            g.ncells = (uint32_t)nc;
-            total += nc;
+            const unsigned long long carry = s_carry;
+            if (s < nseg) grid[s].cell_base = (uint32_t)(carry + s_scan[t] - nc);
+            __syncthreads();
+            if (t == kGridThreads - 1) s_carry = carry + s_scan[t];
+            __syncthreads();
         }
+        total = s_carry;
+        __syncthreads();
         if (total <= capacity) break;
         scale *= 1.25;
     }
-    if (total > capacity) { st[0].bad |= 1; total = capacity; }
-    *total_out = (uint32_t)total;
+    if (t == 0) {
+        if (total > capacity) { st[0].bad |= 1; total = capacity; }
+        *total_out = (uint32_t)total;
+    }
 }
 
 __device__ __forceinline__ int cell_coord(const SegGrid& g, int a, double v) {

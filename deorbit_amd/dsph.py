@@ -313,3 +313,24 @@ def make_system(ic: dict, precision: int = MIXED, seed_damage: int | None = 1234
     if p.get("damage_on") and seed_damage is not None:
         s.seed_damage(ic["dx"], seed_damage)
     return s
+
+
+def make_batch(ic_list: list, precision: int = MIXED, seed_damage: int | None = 12345) -> System:
+    """One batched context holding every IC of `ic_list` as its own segment (its own
+    parameters, domain, time step and density rounds), uploaded in one call and
+    seeded per segment like separate runs.  All ICs must share one material table."""
+    mats = ic_list[0]["materials"]
+    s = System([ic["params"] for ic in ic_list], mats, precision)
+    cat = lambda k: np.concatenate([np.asarray(ic[k]) for ic in ic_list])
+    seg = np.concatenate([np.full(len(ic["mass"]), k, np.int32) for k, ic in enumerate(ic_list)])
+    s.add(cat("pos"), cat("vel"), cat("mass"), cat("u"), cat("mat"), cat("fixed"), seg)
+    if seed_damage is not None and any(ic["params"].get("damage_on") for ic in ic_list):
+        # every segment draws its flaws from its own mt19937(seed), as separate runs do
+        dxs = {float(ic["dx"]) for ic in ic_list}
+        if len(dxs) == 1:
+            s.seed_damage(dxs.pop(), seed_damage, seg=-1)
+        else:
+            for k, ic in enumerate(ic_list):
+                s.seed_damage(ic["dx"], seed_damage, seg=k)
+    return s
+

@@ -183,6 +183,31 @@ def ensemble_instance(member: dict, steps: int = 500):
                   name=f"ensemble{member['index']:03d}", steps=steps)
 
 
+def impact_region(ic: dict, nmax: int) -> dict:
+    """The `nmax` particles nearest (in max-norm) to the impactor's lowest point,
+    where it meets the target: a bounded sample of a large impact that keeps its
+    contact zone, part impactor and part target (used to time the CPU solver on a
+    share of a workload too large for it)."""
+    n = len(ic["mass"])
+    if n <= nmax:
+        return ic
+    imp = np.flatnonzero(ic["mat"] == M.MAT_IMPACTOR)
+    c = ic["pos"][imp[np.argmin(ic["pos"][imp, 2])]] if len(imp) else ic["pos"].mean(axis=0)
+    d = np.abs(ic["pos"] - c).max(axis=1)
+    keep = np.sort(np.argpartition(d, nmax)[:nmax])
+    out = dict(ic)
+    for k in ("pos", "vel", "mass", "u", "mat", "fixed"):
+        out[k] = np.ascontiguousarray(np.asarray(ic[k])[keep])
+    out["meta"] = dict(ic["meta"], name=ic["meta"]["name"] + f"-region{nmax}")
+    return out
+
+
+def ensemble(count: int = 256, steps: int = 500, seed0: int = 3000):
+    """The batched ensemble: `count` independent iron-on-granite impacts (strength and
+    damage on) drawn by ensemble_members; one IC dict per instance."""
+    return [ensemble_instance(m, steps) for m in ensemble_members(count, seed0)]
+
+
 def _arange(start, stop, step):
     # float stepping exactly as boundary_daemon.py:113-117 does it (accumulated sums)
     out, x = [], start

@@ -137,3 +137,16 @@ def test_grazing_ic_reproduces_the_campaign_geometry(tmp_path):
     formats.write_ic(p, small)
     back = formats.read_ic(p)
     assert np.array_equal(back["fixed"], small["fixed"]) and np.array_equal(back["mat"], small["mat"])
+
+
+def test_ensemble_batch_and_impact_region_sample():
+    E = ics.ensemble(8, 500)
+    assert len(E) == 8 and all(e["meta"]["steps"] == 500 for e in E)
+    assert all(8000 * 0.8 < len(e["mass"]) < 200000 * 1.2 for e in E)
+    ic = ics.config1()
+    r = ics.impact_region(ic, 200000)
+    assert len(r["mass"]) == 200000
+    imp = int((r["mat"] == 1).sum())
+    assert 0.2 * 200000 < imp < 0.8 * 200000          # both impactor and target in the sample
+    assert ics.impact_region(ic, 10 ** 7) is ic
+
