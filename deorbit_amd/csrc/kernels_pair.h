@@ -258,12 +258,14 @@ __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p
         L.frame(dx, dy, dz, cw, bx, by, bz);
         if (!L.emu) {
             uint32_t k = rg.x;
-            for (; k + 1 < rg.y; k += 2) {       // two loads in flight
-                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1];
+            for (; k + 3 < rg.y; k += 4) {       // four loads in flight
+                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1], q2 = w.dens_rec[k + 2], q3 = w.dens_rec[k + 3];
                 s += L.term(q0, bx, by, bz);
                 s += L.term(q1, bx, by, bz);
+                s += L.term(q2, bx, by, bz);
+                s += L.term(q3, bx, by, bz);
             }
-            if (k < rg.y) s += L.term(w.dens_rec[k], bx, by, bz);
+            for (; k < rg.y; k++) s += L.term(w.dens_rec[k], bx, by, bz);
         } else {
             for (uint32_t k = rg.x; k < rg.y; k++) {
                 if (!oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
@@ -443,8 +445,7 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
                 const uint2 rg = w.table[key];
                 R bx, by, bz;
                 L.frame(dx, dy, dz, cw, bx, by, bz);
-                for (uint32_t k = rg.x; k < rg.y; k++) {
-                    const V4 q = w.dens_rec[k];
+                auto visit = [&](uint32_t k, const V4& q) {
                     const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
                     const R r2 = rx * rx + ry * ry + rz * rz;
                     if (r2 < rb2) {
@@ -458,9 +459,16 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
                         }
                         cnt++;
                     }
-                    if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
+                    if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) return;
                     s += L.term(q, bx, by, bz);
+                };
+                uint32_t k = rg.x;
+                for (; k + 1 < rg.y; k += 2) {       // two loads in flight
+                    const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1];
+                    visit(k, q0);
+                    visit(k + 1, q1);
                 }
+                if (k < rg.y) visit(k, w.dens_rec[k]);
             });
             if (cnt > (uint32_t)cap) {
                 cnt = kNbrOverflow;
@@ -670,12 +678,20 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                     } else {
                         bx = F.pi.x; by = F.pi.y; bz = F.pi.z;
                     }
-                    for (uint32_t k = rg.x; k < rg.y; k++) {
+                    // candidate k: the other records are read only for pairs in range
+                    auto cand = [&](uint32_t k, const typename Vec4<R>::T& q) {
                         R rx, ry, rz, r2, th;
-                        if (!F.in_range(w.frec_pos[k], bx, by, bz, rx, ry, rz, r2, th)) continue;
+                        if (!F.in_range(q, bx, by, bz, rx, ry, rz, r2, th)) return;
                         F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
                                w.frec_sb[k]);
+                    };
+                    uint32_t k = rg.x;
+                    for (; k + 1 < rg.y; k += 2) {   // two position records in flight
+                        const typename Vec4<R>::T q0 = w.frec_pos[k], q1 = w.frec_pos[k + 1];
+                        cand(k, q0);
+                        cand(k + 1, q1);
                     }
+                    if (k < rg.y) cand(k, w.frec_pos[k]);
                 });
             }
             dt_i = F.store(c, w, i, info_mat(inf), p);
