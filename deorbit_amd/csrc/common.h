@@ -91,6 +91,8 @@ struct Work {
     V4* frec_sa;                     // S*dscale/rho^2 : xx yy zz xy
     V4* frec_sb;                     // S*dscale/rho^2 : xz yz, m/rho, 0
     uint32_t* cellc;                 // packed (cx | cy << 10 | cz << 20) of the sort cell
+    uint32_t* dev_of;                // IC index -> slot in the order before this step's sort
+    uint32_t* perm;                  // new slot -> slot in the order before this step's sort
     uint2* table;                    // per sort cell: [start, end) in the current order
     float* cell_hmax;                // per sort cell: upper bound of h over its particles
     // Neighbour lists, built by the first density iteration of a step and read

@@ -142,6 +142,8 @@ class Engine : public EngineBase {
         w.frec_pos = dalloc<V4>(n_); w.frec_vel = dalloc<V4>(n_); w.frec_th = dalloc<V4>(n_);
         w.frec_sa = dalloc<V4>(n_); w.frec_sb = dalloc<V4>(n_);
         w.cellc = dalloc<uint32_t>(n_);
+        w.dev_of = dalloc<uint32_t>(n_);
+        w.perm = dalloc<uint32_t>(n_);
         nparts_max_ = (int)blocks_for(n_, 64) + 1;
         w.parts = dalloc<BlockPart>(nparts_max_);
         alloc_table();
@@ -420,7 +422,7 @@ class Engine : public EngineBase {
             // Outside a step every per-particle field is state the caller may read
             // or feed back (System keeps them between calls), so carry them through
             // the re-sort as well.
-            const uint32_t* perm = vals_[ka];
+            const uint32_t* perm = w_.perm;
             R* scratch = (R*)stage_;
             Work<R>& w = w_;
             for (R* f : {w.ax, w.ay, w.az, w.rho, w.P, w.cs, w.csig, w.Pf, w.Rart, w.dscale, w.sigmax,
@@ -583,6 +585,24 @@ class Engine : public EngineBase {
         HIPC(hipMemcpyAsync(h.data(), d_st_, nseg_ * sizeof(SegState), hipMemcpyDeviceToHost, stream_));
         sync();
         return h;
+    }
+
+    // Restore a segment's clock: simulated time, step count and the CFL step
+    // the last force pass left (so a restarted run takes the same next step).
+    void set_clock(int seg, double t, int64_t steps, double dt_next) {
+        auto st = read_state();
+        for (int s = 0; s < nseg_; s++) {
+            if (seg >= 0 && s != seg) continue;
+            st[s].t = t;
+            st[s].steps = steps;
+            unsigned long long b;
+            std::memcpy(&b, &dt_next, 8);
+            st[s].dt_next = b;
+        }
+        HIPC(hipMemcpyAsync(d_st_, st.data(), nseg_ * sizeof(SegState), hipMemcpyHostToDevice, stream_));
+        sync();
+        if (dt_next > 0.0) have_dt_ = true;
+        release_graphs();
     }
 
     int check_health() {
@@ -1220,6 +1240,12 @@ int dsph_run(dsph_ctx* c, int32_t nsteps, double dt, int32_t use_graph) {
            e.run(nsteps, dt, use_graph != 0));
 }
 int dsph_synchronize(dsph_ctx* c) { CTX_OP(e.sync(); e.check_health()); }
+int dsph_set_clock(dsph_ctx* c, int32_t seg, double t, int64_t steps, double dt_next) {
+    CTX_OP(require(seg < (int)c->segp.size(), DSPH_E_ARG, "segment out of range");
+           require(std::isfinite(t) && steps >= 0 && std::isfinite(dt_next) && dt_next >= 0, DSPH_E_ARG,
+                   "time, step count and dt must be finite and non-negative");
+           e.set_clock(seg, t, steps, dt_next));
+}
 int dsph_diagnostics(dsph_ctx* c, int32_t seg, dsph_diag* out) {
     CTX_OP(require(out != nullptr && seg >= 0 && seg < (int)c->segp.size(), DSPH_E_ARG, "bad segment or null output");
            e.diagnostics(seg, out));

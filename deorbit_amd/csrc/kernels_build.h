@@ -180,8 +180,12 @@ __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t*
         uint32_t key = g.cell_base + g.ncells - 1u;              // park cell
         if (isfinite(x) && isfinite(y) && isfinite(z))
             key = cell_key(g, cell_coord(g, 0, x), cell_coord(g, 1, y), cell_coord(g, 2, z));
-        keys[i] = key;
-        vals[i] = (uint32_t)i;
+        // the sort runs over IC order, so particles of one cell end up in IC
+        // order whatever order the previous step left them in
+        const uint32_t o = c.orig[i];
+        keys[o] = key;
+        vals[o] = o;
+        w.dev_of[o] = (uint32_t)i;
     }
     const uint32_t total = *total_cells;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -323,7 +327,8 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
     int seg = 0;
     double hv = 0.0;
     if (in) {
-        const uint32_t s = vals[i];
+        const uint32_t s = w.dev_of[vals[i]];   // current slot of the particle sorted to i
+        w.perm[i] = s;
         const uint32_t inf = src.info[s];
         seg = info_seg(inf);
         const double x = src.x[s], y = src.y[s], z = src.z[s], m = src.mass[s];

@@ -33,7 +33,7 @@ DTYPE = {F_MAT: np.int32, F_SEG: np.int32, F_FIXED: np.uint8}
 EXPORTS = ["dsph_version", "dsph_last_error", "dsph_create", "dsph_batch_create", "dsph_destroy",
            "dsph_upload", "dsph_count", "dsph_seed_damage", "dsph_get_field", "dsph_set_field",
            "dsph_init", "dsph_compute_density_h", "dsph_compute_forces", "dsph_grow_damage",
-           "dsph_timestep", "dsph_step", "dsph_step_dts", "dsph_run", "dsph_synchronize",
+           "dsph_timestep", "dsph_step", "dsph_step_dts", "dsph_run", "dsph_synchronize", "dsph_set_clock",
            "dsph_diagnostics", "dsph_set_profiling", "dsph_get_profile", "dsph_reset_profile",
            "dsph_get_tuning", "dsph_set_tuning", "dsph_autotune"]
 
@@ -114,6 +114,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.dsph_step.argtypes = [vp, C.c_double]
     lib.dsph_step_dts.argtypes = [vp, dp]
     lib.dsph_run.argtypes = [vp, C.c_int32, C.c_double, C.c_int32]
+    lib.dsph_set_clock.argtypes = [vp, C.c_int32, C.c_double, C.c_int64, C.c_double]
     lib.dsph_diagnostics.argtypes = [vp, C.c_int32, C.POINTER(Diag)]
     lib.dsph_set_profiling.argtypes = [vp, C.c_int32]
     lib.dsph_get_profile.argtypes = [vp, C.POINTER(Profile)]
@@ -264,6 +265,27 @@ class System:
 
     def synchronize(self):
         self._check(self.lib.dsph_synchronize(self.h))
+
+    def set_clock(self, t: float, steps: int, dt_next: float, seg: int = -1):
+        self._check(self.lib.dsph_set_clock(self.h, seg, t, steps, dt_next))
+
+    # --- checkpoint / restart (run_ic format, lossless extension) ---------------
+    CHECKPOINT_FIELDS = ("pos", "vel", "mass", "u", "mat", "fixed", "D", "eps_act", "S", "hh", "acc", "dudt",
+                         "dSdt")
+
+    def checkpoint_state(self) -> dict:
+        """Every field a bit-exact restart needs, IC order, plus the clock of segment 0."""
+        st = self.state(list(self.CHECKPOINT_FIELDS))
+        d = self.diagnostics(0)
+        st.update(t=d["t"], steps=d["steps"], dt_next=d["dt_next"])
+        return st
+
+    def restore_state(self, st: dict):
+        """Load a checkpoint_state / read_checkpoint dict into a system that holds the
+        same particles (same count, IC order)."""
+        for f in ("D", "eps_act", "S", "hh", "acc", "dudt", "dSdt"):
+            self.set(f, st[f])
+        self.set_clock(st["t"], int(st.get("steps", 0)), float(st.get("dt_next", 0.0)))
 
     # --- diagnostics / profiling / tuning -----------------------------------
     def diagnostics(self, seg: int = 0) -> dict:

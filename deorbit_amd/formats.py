@@ -9,8 +9,10 @@
   written to <file>.tmp and renamed.
 * extended checkpoint (this project): the checkpoint rows followed by n rows
   [h ax ay az dudt dSdt_xx dSdt_xy dSdt_xz dSdt_yy dSdt_yz dSdt_zz] and a trailer
-  [magic steps], so a restart continues bit for bit (the reference checkpoint
-  omits h, acc and the previous derivatives, so its restart re-runs init()).
+  [magic steps dt_next] (dt_next: the CFL step of the last force evaluation), so a
+  restart continues bit for bit (the reference checkpoint omits h, acc and the
+  previous derivatives, so its restart re-runs init()).  Files with the older
+  two-value trailer [magic0 steps] are still read.
 
 Tensors inside the files use the reference's {xx, xy, xz, yy, yz, zz} order;
 the in-memory arrays of this package use the Sym3 order {xx, yy, zz, xy, xz, yz}.
@@ -24,7 +26,8 @@ import numpy as np
 # file order {xx,xy,xz,yy,yz,zz} -> memory order {xx,yy,zz,xy,xz,yz} and back
 FILE_TO_MEM = [0, 3, 5, 1, 2, 4]
 MEM_TO_FILE = [0, 3, 4, 1, 5, 2]
-EXT_MAGIC = 20241203.0
+EXT_MAGIC = 20241204.0           # trailer [magic steps dt_next]
+EXT_MAGIC_V0 = 20241203.0        # trailer [magic steps]
 
 
 def write_ic(path: str, ic: dict) -> None:
@@ -94,7 +97,7 @@ def _atomic_write(path: str, chunks) -> None:
 
 
 def write_checkpoint(path: str, st: dict, lo, hi, t: float, dx: float, extended: bool = False,
-                     steps: int = 0) -> None:
+                     steps: int = 0, dt_next: float = 0.0) -> None:
     """``st`` holds IC-order arrays: pos vel mass u mat fixed D eps_act S (memory order),
     and for ``extended`` also hh acc dudt dSdt."""
     n = len(st["mass"])
@@ -116,7 +119,7 @@ def write_checkpoint(path: str, st: dict, lo, hi, t: float, dx: float, extended:
         ext[:, 1:4] = st["acc"]
         ext[:, 4] = st["dudt"]
         ext[:, 5:11] = np.asarray(st["dSdt"]).reshape(n, 6)[:, MEM_TO_FILE]
-        chunks += [ext, [EXT_MAGIC, steps]]
+        chunks += [ext, [EXT_MAGIC, steps, dt_next]]
     _atomic_write(path, chunks)
 
 
@@ -135,10 +138,15 @@ def read_checkpoint(path: str) -> dict:
                mat=rows[:, 8].astype(np.int32), fixed=(rows[:, 9] != 0).astype(np.uint8), D=rows[:, 10].copy(),
                eps_act=rows[:, 11].copy(), S=rows[:, 12:18][:, FILE_TO_MEM].copy(), extended=False)
     rest = raw[9 + 18 * n:]
-    if rest.size == 11 * n + 2 and rest[-2] == EXT_MAGIC:
+    trailer = None
+    if rest.size == 11 * n + 3 and rest[-3] == EXT_MAGIC:
+        trailer = dict(steps=int(rest[-2]), dt_next=float(rest[-1]))
+    elif rest.size == 11 * n + 2 and rest[-2] == EXT_MAGIC_V0:
+        trailer = dict(steps=int(rest[-1]), dt_next=0.0)
+    if trailer is not None:
         ext = rest[:11 * n].reshape(n, 11)
         out.update(hh=ext[:, 0].copy(), acc=ext[:, 1:4].copy(), dudt=ext[:, 4].copy(),
-                   dSdt=ext[:, 5:11][:, FILE_TO_MEM].copy(), steps=int(rest[-1]), extended=True)
+                   dSdt=ext[:, 5:11][:, FILE_TO_MEM].copy(), extended=True, **trailer)
     elif rest.size:
         raise ValueError(f"{path}: unrecognised trailing data")
     return out

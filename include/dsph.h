@@ -157,6 +157,13 @@ int dsph_step_dts(dsph_ctx* ctx, const double* dts);
  * gpu_ic.cpp:128-144 without its per-step host reads. */
 int dsph_run(dsph_ctx* ctx, int32_t nsteps, double dt, int32_t use_graph);
 int dsph_synchronize(dsph_ctx* ctx);
+/* Set the clock of segment seg (< 0: all): simulated time, step count and the
+ * CFL step of the last force evaluation, as a checkpoint records them.  With
+ * every field of DSPH_F_* restored as well (including ACC, DUDT and DSDT, the
+ * derivatives the next step integrates with), the next dsph_step / dsph_run
+ * continues bit for bit where the checkpointed run stopped.  Replaces the
+ * restart path of run_ic.cpp:76-102 (which restores t from the header). */
+int dsph_set_clock(dsph_ctx* ctx, int32_t seg, double t, int64_t steps, double dt_next);
 
 /* ---- diagnostics, profiling, tuning ---------------------------------- */
 int dsph_diagnostics(dsph_ctx* ctx, int32_t seg, dsph_diag* out);

@@ -113,6 +113,14 @@ class System {
         check(dsph_get_field(ctx_, DSPH_F_MAT, out.data()));
         return out;
     }
+    std::vector<uint8_t> get_fixed() {
+        flush();
+        std::vector<uint8_t> out((size_t)n());
+        check(dsph_get_field(ctx_, DSPH_F_FIXED, out.data()));
+        return out;
+    }
+    // restore a checkpointed clock (no reference counterpart; see dsph_set_clock)
+    void set_clock(double t, int64_t steps, double dt_next) { flush(); check(dsph_set_clock(ctx_, -1, t, steps, dt_next)); }
     dsph_ctx* handle() { flush(); return ctx_; }
 
   private:

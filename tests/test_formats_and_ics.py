@@ -116,12 +116,18 @@ def test_checkpoint_roundtrip_plain_and_extended(tmp_path):
     assert back["t"] == 0.25 and back["dx"] == 10.0 and not back["extended"]
     for k in ("pos", "vel", "mass", "u", "D", "eps_act", "S"):
         assert np.array_equal(back[k], st[k])
-    formats.write_checkpoint(p, st, (0, 0, 0), (1, 2, 3), 0.5, 10.0, extended=True, steps=17)
+    formats.write_checkpoint(p, st, (0, 0, 0), (1, 2, 3), 0.5, 10.0, extended=True, steps=17, dt_next=3.5e-4)
+    assert os.path.getsize(p) == 8 * (9 + 18 * n + 11 * n + 3)
     back = formats.read_checkpoint(p)
-    assert back["extended"] and back["steps"] == 17
+    assert back["extended"] and back["steps"] == 17 and back["dt_next"] == 3.5e-4
     for k in ("hh", "acc", "dudt", "dSdt"):
         assert np.array_equal(back[k], st[k])
     assert not os.path.exists(p + ".tmp")
+    # files with the earlier two-value trailer still read (no dt_next)
+    raw = np.fromfile(p, "<f8")
+    np.concatenate([raw[:-3], [formats.EXT_MAGIC_V0, 17.0]]).astype("<f8").tofile(p)
+    back = formats.read_checkpoint(p)
+    assert back["extended"] and back["steps"] == 17 and back["dt_next"] == 0.0
 
 
 def test_grazing_ic_reproduces_the_campaign_geometry(tmp_path):

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from conftest import scale_err
-from deorbit_amd import dsph, ics
+from deorbit_amd import dsph, formats, ics
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -79,6 +79,38 @@ def test_neighbour_lists_give_the_cell_walk_results(precision, gpu):
                 assert np.array_equal(st[f], ref[f]), (name, f, scale_err(st[f], ref[f]))
             else:
                 assert scale_err(st[f], ref[f]) < 1e-6, (name, f, scale_err(st[f], ref[f]))
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_checkpoint_restart_is_bit_exact(precision, gpu, tmp_path):
+    """Six steps straight through equal three steps, an extended checkpoint written
+    to disk, a fresh context restored from it, and three more steps: bit for bit
+    (the per-step sort orders particles of a cell by IC index, so the restored
+    context sums in the same order)."""
+    ic = ics.tiny_impact()
+    a = dsph.make_system(ic, precision)
+    a.init()
+    a.run(6, 0.0, True)
+    ref, dref = a.state(list(FIELDS)), a.diagnostics()
+    a.close()
+    b = dsph.make_system(ic, precision)
+    b.init()
+    b.run(3, 0.0, True)
+    st = b.checkpoint_state()
+    b.close()
+    path = str(tmp_path / "ckpt.bin")
+    formats.write_checkpoint(path, st, ic["lo"], ic["hi"], st["t"], ic["dx"], extended=True, steps=st["steps"],
+                             dt_next=st["dt_next"])
+    ck = formats.read_checkpoint(path)
+    c = dsph.System(ic["params"], ic["materials"], precision)
+    c.add(ck["pos"], ck["vel"], ck["mass"], ck["u"], ck["mat"], ck["fixed"])
+    c.restore_state(ck)
+    c.run(3, 0.0, True)
+    got, dgot = c.state(list(FIELDS)), c.diagnostics()
+    c.close()
+    for f in FIELDS:
+        assert np.array_equal(got[f], ref[f]), (f, scale_err(got[f], ref[f]))
+    assert dgot["t"] == dref["t"] and dgot["steps"] == dref["steps"] == 6
 
 
 def test_tuning_changes_speed_not_results_beyond_roundoff(gpu):
