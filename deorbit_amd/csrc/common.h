@@ -32,10 +32,14 @@ struct SegPar {
 // Per-segment sort grid and oracle-emulation grid (written on device).
 struct SegGrid {
     double lo[3];
-    double cw, inv_cw;
+    double cw[3], inv_cw[3];     // cell width per axis (periodic axes: L / dim)
     int32_t dim[3];
     int32_t bdim[3];
     uint32_t cell_base, ncells;
+    // periodic axes: the grid spans the domain [dlo, dhi) of length L with
+    // dim whole cells, so the image of a cell is exactly L away
+    int32_t per[3];
+    double L[3];
     // cell grid the reference would use this density round (sph.hpp:147-157)
     double ocw[3];
     int32_t onc[3];

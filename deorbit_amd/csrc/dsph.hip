@@ -1068,9 +1068,10 @@ void validate_params(const dsph_params& p) {
     require(p.eta > 0 && p.h_init > 0 && p.cfl > 0, DSPH_E_ARG, "eta, h_init and cfl must be positive");
     require(p.alpha >= 0 && p.beta >= 0 && p.eps >= 0 && p.alpha_u >= 0 && p.eps_as >= 0, DSPH_E_ARG,
             "alpha, beta, eps, alpha_u and eps_as must be non-negative");
-    require(!(p.periodic[0] || p.periodic[1] || p.periodic[2]), DSPH_E_ARG,
-            "periodic boundaries are not supported by the device path");
-    for (int a = 0; a < 3; a++) require(std::isfinite(p.lo[a]) && std::isfinite(p.hi[a]), DSPH_E_ARG, "domain bounds must be finite");
+    for (int a = 0; a < 3; a++) {
+        require(std::isfinite(p.lo[a]) && std::isfinite(p.hi[a]), DSPH_E_ARG, "domain bounds must be finite");
+        require(!p.periodic[a] || p.hi[a] > p.lo[a], DSPH_E_ARG, "a periodic axis needs hi > lo");
+    }
 }
 
 void ensure_committed(dsph_ctx* c) {

@@ -120,6 +120,19 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
                     ext[a] = h - l;
                     emax = ext[a] > emax ? ext[a] : emax;
                 }
+                // periodic axes span the domain instead of the particle bbox
+                // (positions are not wrapped; cell_coord clamps strays into the
+                // edge cells as the reference's binof does)
+                int per[3];
+                for (int a = 0; a < 3; a++) {
+                    const double L = par[s].dhi[a] - par[s].dlo[a];
+                    per[a] = par[s].per[a] && L > 0.0;
+                    if (per[a]) {
+                        lo[a] = par[s].dlo[a];
+                        ext[a] = L;
+                        emax = ext[a] > emax ? ext[a] : emax;
+                    }
+                }
                 double cw = cell_factor * par[s].h_init * scale;
                 if (emax / cw > 1000.0) cw = emax / 1000.0;         // at most 1001 cells per axis
                 // 64 cells per brick plus one trailing "park" cell that collects
@@ -127,13 +140,24 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
                 nc = 64;
                 for (int a = 0; a < 3; a++) {
                     g.lo[a] = lo[a];
-                    g.dim[a] = (int)floor(ext[a] / cw) + 1;
+                    g.per[a] = per[a];
+                    if (per[a]) {
+                        // whole cells of at least the nominal width tile the period
+                        int m = (int)floor(ext[a] / cw);
+                        m = m < 1 ? 1 : m;
+                        g.dim[a] = m;
+                        g.L[a] = ext[a];
+                        g.cw[a] = ext[a] / m;
+                    } else {
+                        g.dim[a] = (int)floor(ext[a] / cw) + 1;
+                        g.L[a] = 0.0;
+                        g.cw[a] = cw;
+                    }
+                    g.inv_cw[a] = 1.0 / g.cw[a];
                     g.bdim[a] = (g.dim[a] + 3) / 4;
                     nc *= (unsigned long long)g.bdim[a];
                 }
                 nc += 1;
-                g.cw = cw;
-                g.inv_cw = 1.0 / cw;
                 g.ncells = (uint32_t)nc;
             }
             // inclusive scan of the chunk's cell counts
@@ -163,7 +187,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
 }
 
 __device__ __forceinline__ int cell_coord(const SegGrid& g, int a, double v) {
-    double t = floor((v - g.lo[a]) * g.inv_cw);
+    double t = floor((v - g.lo[a]) * g.inv_cw[a]);
     int c = (t < 0.0) ? 0 : (t > (double)(g.dim[a] - 1) ? g.dim[a] - 1 : (int)t);
     return c;
 }
@@ -362,9 +386,9 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
             w.cellc[i] = cellc_pack(cx, cy, cz);
             V4 rec;
             if (sizeof(R) == 4) {
-                rec.x = (R)(x - (g.lo[0] + (double)cx * g.cw));
-                rec.y = (R)(y - (g.lo[1] + (double)cy * g.cw));
-                rec.z = (R)(z - (g.lo[2] + (double)cz * g.cw));
+                rec.x = (R)(x - (g.lo[0] + (double)cx * g.cw[0]));
+                rec.y = (R)(y - (g.lo[1] + (double)cy * g.cw[1]));
+                rec.z = (R)(z - (g.lo[2] + (double)cz * g.cw[2]));
             } else {
                 rec.x = (R)x; rec.y = (R)y; rec.z = (R)z;
             }

@@ -21,9 +21,9 @@ __device__ __forceinline__ void own_offset(const Carried<R>& c, const SegGrid& g
         ox = rec.x; oy = rec.y; oz = rec.z;
     } else {
         const int cx = cc & 1023u, cy = (cc >> 10) & 1023u, cz = cc >> 20;
-        ox = (R)(c.x[i] - (g.lo[0] + (double)cx * g.cw));
-        oy = (R)(c.y[i] - (g.lo[1] + (double)cy * g.cw));
-        oz = (R)(c.z[i] - (g.lo[2] + (double)cz * g.cw));
+        ox = (R)(c.x[i] - (g.lo[0] + (double)cx * g.cw[0]));
+        oy = (R)(c.y[i] - (g.lo[1] + (double)cy * g.cw[1]));
+        oz = (R)(c.z[i] - (g.lo[2] + (double)cz * g.cw[2]));
     }
 }
 
@@ -34,6 +34,55 @@ __device__ __forceinline__ R cell_gap(int d, R o, R cw) {
     const R g = lo > hi ? lo : hi;
     return g > R(0) ? g : R(0);
 }
+
+// Minimum-image separation on periodic axes (sph.hpp:130-136):
+// d -= L round(d / L), round half away from zero.
+__device__ __forceinline__ float d_round(float x) { return roundf(x); }
+__device__ __forceinline__ double d_round(double x) { return round(x); }
+template <class R>
+__device__ __forceinline__ void min_image(const SegGrid& g, R& rx, R& ry, R& rz) {
+    if (g.per[0]) { const R L = (R)g.L[0]; rx -= L * d_round(rx / L); }
+    if (g.per[1]) { const R L = (R)g.L[1]; ry -= L * d_round(ry / L); }
+    if (g.per[2]) { const R L = (R)g.L[2]; rz -= L * d_round(rz / L); }
+}
+
+// The cells a stencil walk visits along one axis.  Offset d names the cell
+// c + d, wrapped on a periodic axis (whose cells tile the period exactly, so a
+// wrapped cell's image sits d cell widths away); the frame shift d * cw is the
+// corner of the visited image relative to the own cell's corner.  A periodic
+// axis with fewer cells than the stencil is walked once over all its cells
+// (offsets -(dim-1)/2 .. dim-1-(dim-1)/2) without culling.
+template <class R>
+struct AxisWalk {
+    int c, dim, lo, hi, per;
+    bool cull;
+    R o, cw;
+    __device__ __forceinline__ void init(const SegGrid& g, int a, int cc, R oo, double reach) {
+        c = cc; dim = g.dim[a]; per = g.per[a]; o = oo; cw = (R)g.cw[a];
+        int rad = (int)ceil(reach * g.inv_cw[a]);
+        rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
+        cull = true;
+        lo = -rad; hi = rad;
+        if (per && 2 * rad + 1 > dim) {
+            lo = -((dim - 1) / 2);
+            hi = dim - 1 - (dim - 1) / 2;
+            cull = false;
+        }
+    }
+    // false when offset d leaves a non-periodic axis
+    __device__ __forceinline__ bool cell(int d, int& n, R& shift, R& gap) const {
+        n = c + d;
+        if (per) {
+            if (n < 0) n += dim;
+            else if (n >= dim) n -= dim;
+        } else if ((unsigned)n >= (unsigned)dim) {
+            return false;
+        }
+        shift = (R)d * cw;
+        gap = cull ? cell_gap(d, o, cw) : R(0);
+        return true;
+    }
+};
 
 // Fast reciprocal / square root: hardware approximations in FP32 (~1 ulp),
 // IEEE operations in FP64.
@@ -154,14 +203,25 @@ __device__ __forceinline__ uint32_t quad_get(const uint4& v, int s) {
 
 // Position of particle i relative to the frame of neighbour k's sort cell:
 // FP32 path, own-cell offset shifted by the cell delta (so the pair separation
-// is formed from numbers below one cell width); FP64 path, absolute position.
+// is formed from numbers below one cell width), taking on a periodic axis the
+// image the stencil walk uses; FP64 path, absolute position.
 template <class R>
-__device__ __forceinline__ void rel_base(const Work<R>& w, uint32_t k, int cx, int cy, int cz, R cw, R ox, R oy,
-                                         R oz, R ax, R ay, R az, R& bx, R& by, R& bz) {
+__device__ __forceinline__ void rel_base(const Work<R>& w, const SegGrid& g, uint32_t k, int cx, int cy, int cz,
+                                         R ox, R oy, R oz, R ax, R ay, R az, R& bx, R& by, R& bz) {
     if (sizeof(R) == 4) {
         const uint32_t ck = w.cellc[k];
-        const int dx = (int)(ck & 1023u) - cx, dy = (int)((ck >> 10) & 1023u) - cy, dz = (int)(ck >> 20) - cz;
-        bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
+        int d[3] = {(int)(ck & 1023u) - cx, (int)((ck >> 10) & 1023u) - cy, (int)(ck >> 20) - cz};
+        R sh[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (g.per[a]) {
+                const int m = g.dim[a], lo = -((m - 1) / 2), hi = m - 1 - (m - 1) / 2;
+                if (d[a] < lo) d[a] += m;
+                else if (d[a] > hi) d[a] -= m;
+            }
+            sh[a] = (R)d[a] * (R)g.cw[a];
+        }
+        bx = ox - sh[0]; by = oy - sh[1]; bz = oz - sh[2];
     } else {
         bx = ax; by = ay; bz = az;
     }
@@ -176,10 +236,14 @@ struct DensLane {
     R h, ox, oy, oz, two_h, th2, inv_h, sig;
     int cx, cy, cz;
     bool emu;              // 2h may exceed the reference's cell width this round
+    bool per;              // some axis of the segment is periodic
     int ob[3];             // the reference's cell of i (emulation only)
+    const SegGrid* gp;
 
     __device__ void setup(const Carried<R>& c, const SegPar& p, const SegGrid& g, int64_t i, uint32_t cc) {
         cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
+        gp = &g;
+        per = g.per[0] || g.per[1] || g.per[2];
         two_h = R(2) * h;
         th2 = two_h * two_h;
         inv_h = R(1) / h;
@@ -195,16 +259,17 @@ struct DensLane {
     // contribution of one candidate; FP32 tests r^2 (W is continuous at 2h),
     // FP64 tests r < 2h exactly as the reference does
     __device__ __forceinline__ R term(const V4& q, R bx, R by, R bz) const {
-        const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+        R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+        if (per) min_image(*gp, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R rr = f_sqrt(r2);
         const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
         return in ? q.w * (sig * m4_shape(rr * inv_h)) : R(0);
     }
-    // frame of the sort cell at stencil offset (dx, dy, dz)
-    __device__ __forceinline__ void frame(int dx, int dy, int dz, R cw, R& bx, R& by, R& bz) const {
+    // frame of the visited sort cell (shifts from for_cells)
+    __device__ __forceinline__ void frame(R sx, R sy, R sz, R& bx, R& by, R& bz) const {
         if (sizeof(R) == 4) {
-            bx = ox - (R)dx * cw; by = oy - (R)dy * cw; bz = oz - (R)dz * cw;
+            bx = ox - sx; by = oy - sy; bz = oz - sz;
         } else {
             bx = me.x; by = me.y; bz = me.z;
         }
@@ -212,33 +277,35 @@ struct DensLane {
 };
 
 // Visit the sort cells whose nearest point is closer than `reach` to particle
-// i, in lexicographic cell order; fn(key, dx, dy, dz, gap^2) per cell.
+// i, in lexicographic cell order; fn(key, sx, sy, sz, gap^2) per cell, with
+// (sx, sy, sz) the frame shift of the visited cell (see AxisWalk).
 template <class R, class F>
 __device__ __forceinline__ void for_cells(const SegGrid& g, int cx, int cy, int cz, R ox, R oy, R oz, R reach,
                                           F&& fn) {
-    const R cw = (R)g.cw;
     const R reach2 = reach * reach;
-    int rad = (int)ceil((double)reach * g.inv_cw);
-    rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
-    for (int dx = -rad; dx <= rad; dx++) {
-        const int nx = cx + dx;
-        if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
-        const R gx = cell_gap(dx, ox, cw);
+    AxisWalk<R> X, Y, Z;
+    X.init(g, 0, cx, ox, (double)reach);
+    Y.init(g, 1, cy, oy, (double)reach);
+    Z.init(g, 2, cz, oz, (double)reach);
+    for (int dx = X.lo; dx <= X.hi; dx++) {
+        int nx;
+        R sx, gx;
+        if (!X.cell(dx, nx, sx, gx)) continue;
         const R gx2 = gx * gx;
         if (gx2 >= reach2) continue;
-        for (int dy = -rad; dy <= rad; dy++) {
-            const int ny = cy + dy;
-            if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
-            const R gy = cell_gap(dy, oy, cw);
+        for (int dy = Y.lo; dy <= Y.hi; dy++) {
+            int ny;
+            R sy, gy;
+            if (!Y.cell(dy, ny, sy, gy)) continue;
             const R gxy = gx2 + gy * gy;
             if (gxy >= reach2) continue;
-            for (int dz = -rad; dz <= rad; dz++) {
-                const int nz = cz + dz;
-                if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
-                const R gz = cell_gap(dz, oz, cw);
+            for (int dz = Z.lo; dz <= Z.hi; dz++) {
+                int nz;
+                R sz, gz;
+                if (!Z.cell(dz, nz, sz, gz)) continue;
                 const R gg = gxy + gz * gz;
                 if (gg >= reach2) continue;
-                fn(cell_key(g, nx, ny, nz), dx, dy, dz, gg);
+                fn(cell_key(g, nx, ny, nz), sx, sy, sz, gg);
             }
         }
     }
@@ -250,12 +317,11 @@ template <class R>
 __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p, const SegGrid& g,
                           const DensLane<R>& L) {
     using V4 = typename Vec4<R>::T;
-    const R cw = (R)g.cw;
     R s = R(0);
-    for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.two_h, [&](uint32_t key, int dx, int dy, int dz, R) {
+    for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.two_h, [&](uint32_t key, R sx, R sy, R sz, R) {
         const uint2 rg = w.table[key];
         R bx, by, bz;
-        L.frame(dx, dy, dz, cw, bx, by, bz);
+        L.frame(sx, sy, sz, bx, by, bz);
         if (!L.emu) {
             uint32_t k = rg.x;
             for (; k + 3 < rg.y; k += 4) {       // four loads in flight
@@ -377,14 +443,13 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
         L.me = w.dens_rec[i];
         own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
         L.setup(c, p, g, i, cc);
-        const R cw = (R)g.cw;
         const uint32_t nc = w.nbr_cnt[i];
         const int cap = w.nbr_cap;
         R s = R(0);
         auto add = [&](uint32_t k, const V4& q) {
             if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) return;
             R bx, by, bz;
-            rel_base<R>(w, k, L.cx, L.cy, L.cz, cw, L.ox, L.oy, L.oz, L.me.x, L.me.y, L.me.z, bx, by, bz);
+            rel_base<R>(w, g, k, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.me.x, L.me.y, L.me.z, bx, by, bz);
             s += L.term(q, bx, by, bz);
         };
         for (uint32_t e = 0; e < nc; e += 4) {
@@ -428,7 +493,6 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
         R rho = rho_floor<R>();
         if (finite) {
             L.setup(c, p, g, i, cc);
-            const R cw = (R)g.cw;
             const R sk = w.skin_build;
             const R hm = (R)bits_to_double(w.st[seg].hmax[0]);
             const int cap = w.nbr_cap;
@@ -438,15 +502,16 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
             // unused lanes of a last partial quad keep valid indices
             uint4 quad = make_uint4((uint32_t)i, (uint32_t)i, (uint32_t)i, (uint32_t)i);
             for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, sk * (L.h + smax(L.h, hm)),
-                         [&](uint32_t key, int dx, int dy, int dz, R gg) {
+                         [&](uint32_t key, R sx, R sy, R sz, R gg) {
                 const R rb = sk * (L.h + smax(L.h, (R)w.cell_hmax[key]));
                 const R rb2 = rb * rb;
                 if (gg >= rb2) return;
                 const uint2 rg = w.table[key];
                 R bx, by, bz;
-                L.frame(dx, dy, dz, cw, bx, by, bz);
+                L.frame(sx, sy, sz, bx, by, bz);
                 auto visit = [&](uint32_t k, const V4& q) {
-                    const R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+                    R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
+                    if (L.per) min_image(g, rx, ry, rz);
                     const R r2 = rx * rx + ry * ry + rz * rz;
                     if (r2 < rb2) {
                         if (cnt < (uint32_t)cap) {
@@ -512,8 +577,10 @@ struct ForceLane {
     V4 pi, vi, sai, sbi;
     R hi, rho_i, pio, csig_i, rart_i;
     R alpha, beta, eps, eps_as, alpha_u, inv_fdp, ui, Pi;
-    R ox, oy, oz, cw;
+    R ox, oy, oz;
     int cx, cy, cz;
+    bool per;              // some axis of the segment is periodic
+    const SegGrid* gp;
     R L00 = 0, L01 = 0, L02 = 0, L10 = 0, L11 = 0, L12 = 0, L20 = 0, L21 = 0, L22 = 0;
     R ax = 0, ay = 0, az = 0, du = 0;
 
@@ -528,7 +595,8 @@ struct ForceLane {
         alpha = (R)p.alpha; beta = (R)p.beta; eps = (R)p.eps; eps_as = (R)p.eps_as; alpha_u = (R)p.alpha_u;
         inv_fdp = (eps_as > R(0)) ? R(1) / kern_shape(R(1) / (R)p.eta) : R(0);
         cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
-        cw = (R)g.cw;
+        gp = &g;
+        per = g.per[0] || g.per[1] || g.per[2];
         ui = alpha_u > R(0) ? (R)c.u[i] : R(0);
         Pi = alpha_u > R(0) ? w.P[i] : R(0);
     }
@@ -587,6 +655,7 @@ struct ForceLane {
     __device__ __forceinline__ bool in_range(const V4& q, R bx, R by, R bz, R& rx, R& ry, R& rz, R& r2,
                                              R& th) const {
         rx = bx - q.x; ry = by - q.y; rz = bz - q.z;
+        if (per) min_image(*gp, rx, ry, rz);
         r2 = rx * rx + ry * ry + rz * rz;
         th = hi + q.w;
         return r2 < th * th && r2 != R(0);
@@ -668,13 +737,13 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
             own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
             if (isfinite((double)F.ox) && isfinite((double)F.oy) && isfinite((double)F.oz)) {
                 for_cells<R>(g, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.hi + (R)w.st[seg].hmax_final,
-                             [&](uint32_t key, int dx, int dy, int dz, R gg) {
+                             [&](uint32_t key, R sx, R sy, R sz, R gg) {
                     const R reach = F.hi + (R)w.cell_hmax[key];
                     if (gg >= reach * reach) return;
                     const uint2 rg = w.table[key];
                     R bx, by, bz;
                     if (sizeof(R) == 4) {
-                        bx = F.ox - (R)dx * F.cw; by = F.oy - (R)dy * F.cw; bz = F.oz - (R)dz * F.cw;
+                        bx = F.ox - sx; by = F.oy - sy; bz = F.oz - sz;
                     } else {
                         bx = F.pi.x; by = F.pi.y; bz = F.pi.z;
                     }
@@ -733,9 +802,9 @@ __global__ __launch_bounds__(256) void k_force_list(Carried<R> c, Work<R> w, int
                 const V4 a0 = w.frec_sa[k0], a1 = w.frec_sa[k1];
                 const V4 b0 = w.frec_sb[k0], b1 = w.frec_sb[k1];
                 R bx, by, bz, rx, ry, rz, r2, th;
-                rel_base<R>(w, k0, F.cx, F.cy, F.cz, F.cw, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
+                rel_base<R>(w, *F.gp, k0, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
                 if (F.in_range(q0, bx, by, bz, rx, ry, rz, r2, th)) F.body(c, w, k0, rx, ry, rz, r2, th, v0, t0, a0, b0);
-                rel_base<R>(w, k1, F.cx, F.cy, F.cz, F.cw, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
+                rel_base<R>(w, *F.gp, k1, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
                 if (has1 && F.in_range(q1, bx, by, bz, rx, ry, rz, r2, th))
                     F.body(c, w, k1, rx, ry, rz, r2, th, v1, t1, a1, b1);
             };

@@ -183,6 +183,66 @@ def ensemble_instance(member: dict, steps: int = 500):
                   name=f"ensemble{member['index']:03d}", steps=steps)
 
 
+def sod(nx_left: int = 100, ny_left: int = 16, gamma: float = 1.4):
+    """The Sod shock tube of the reference's validation gate (sod.cpp): ideal gas,
+    x in [-0.5, 0.5] (left state rho=1, P=1 on a lattice of spacing a = 0.5/nx_left;
+    right state rho=0.125, P=0.1 on spacing 2a, equal particle masses), periodic in
+    y and z over ny_left * a, particles within 2 h_init of either end frozen.
+    Artificial viscosity alpha=1, beta=2 and Price conductivity alpha_u=1.
+    Defaults give the reference's 28,800 particles."""
+    aL = 0.5 / nx_left
+    aR = 2.0 * aL
+    nx_right, ny_right = nx_left // 2, ny_left // 2
+    Lyz = ny_left * aL
+    eta = 1.3
+    h0 = eta * aL
+    rhoL, rhoR, PL, PR = 1.0, 0.125, 1.0, 0.1
+    m = rhoL * aL ** 3
+    uL = PL / ((gamma - 1.0) * rhoL)
+    uR = PR / ((gamma - 1.0) * rhoR)
+    xfix = 2.0 * h0
+
+    def block(x0, a, nx, ny, u):
+        i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(ny), indexing="ij")
+        pos = np.stack([x0 + (i.ravel() + 0.5) * a, (j.ravel() + 0.5) * a, (k.ravel() + 0.5) * a], axis=1)
+        return pos, np.full(len(pos), u)
+
+    pl, ul = block(-0.5, aL, nx_left, ny_left, uL)
+    pr, ur = block(0.0, aR, nx_right, ny_right, uR)
+    pos = np.concatenate([pl, pr])
+    n = len(pos)
+    fixed = ((pos[:, 0] < -0.5 + xfix) | (pos[:, 0] > 0.5 - xfix)).astype(np.uint8)
+    params = dict(eta=eta, h_init=h0, alpha=1.0, beta=2.0, eps=0.01, alpha_u=1.0, cfl=0.2, eps_as=0.0,
+                  damage_on=False, periodic=(False, True, True), lo=(-0.5, 0.0, 0.0), hi=(0.5, Lyz, Lyz),
+                  g=0.0, damp=1.0)
+    return dict(pos=pos, vel=np.zeros((n, 3)), mass=np.full(n, m), u=np.concatenate([ul, ur]),
+                mat=np.zeros(n, np.int32), fixed=fixed, dx=aL, lo=params["lo"], hi=params["hi"], params=params,
+                materials=[M.ideal(gamma)], meta=dict(name=f"sod{nx_left}x{ny_left}", steps=0, t_end=0.2,
+                                                      states=dict(rhoL=rhoL, PL=PL, rhoR=rhoR, PR=PR, gamma=gamma)))
+
+
+def elastic_wave(dx: float = 0.01, x0: float = -0.6, x1: float = 0.6, Lyz: float = 0.08, v0: float = 1.0,
+                 width: float = 0.06):
+    """The elastic longitudinal wave of the reference's strength gate
+    (strength_test.cpp:11-24,64-85): a basalt bar on a cubic lattice, periodic in
+    y and z, a Gaussian velocity pulse v_x = v0 exp(-x^2/w^2) at rest otherwise,
+    the ends |x| > 0.55 frozen.  The pulse splits into two waves travelling at
+    c_L = sqrt((A + 4G/3) / rho0)."""
+    mat = M.basalt_preset()
+    nx, ny = int(round((x1 - x0) / dx)), int(round(Lyz / dx))
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(ny), indexing="ij")
+    pos = np.stack([x0 + (i.ravel() + 0.5) * dx, (j.ravel() + 0.5) * dx, (k.ravel() + 0.5) * dx], axis=1)
+    n = len(pos)
+    vel = np.zeros((n, 3))
+    vel[:, 0] = v0 * np.exp(-(pos[:, 0] ** 2) / (width * width))
+    fixed = (np.abs(pos[:, 0]) > 0.55).astype(np.uint8)
+    params = M.default_params(dx, (x0, 0.0, 0.0), (x1, Lyz, Lyz), False, periodic=(False, True, True))
+    c_L = math.sqrt((mat["A"] + 4.0 / 3.0 * mat["G"]) / mat["rho0"])
+    return dict(pos=pos, vel=vel, mass=np.full(n, mat["rho0"] * dx ** 3), u=np.zeros(n), mat=np.zeros(n, np.int32),
+                fixed=fixed, dx=dx, lo=params["lo"], hi=params["hi"], params=params, materials=[mat],
+                meta=dict(name="elastic_wave", steps=0, t_end=6.0e-5, c_L=c_L))
+
+
 def impact_region(ic: dict, nmax: int) -> dict:
     """The `nmax` particles nearest (in max-norm) to the impactor's lowest point,
     where it meets the target: a bounded sample of a large impact that keeps its

@@ -231,3 +231,19 @@ def test_damage_growth_law():
     rate = 0.4 * math.sqrt(E / 2680.0) / (0.5 * prm["h_init"] / prm["eta"])
     assert D[0] == pytest.approx(min(1.0, (0.5 + rate * dt) ** 3), rel=1e-14)
     assert D[1] == 0.125 and D[2] == 0.0
+
+
+def test_exact_riemann_solver_reproduces_the_sod_star_state():
+    # Toro (1997), Table 4.2, test 1: p* = 0.30313, u* = 0.92745,
+    # rho*_L = 0.42632, rho*_R = 0.26557; right shock speed 1.75216
+    from riemann import RiemannIdealGas
+    rs = RiemannIdealGas(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    assert abs(rs.p_star - 0.30313) < 5e-6 and abs(rs.u_star - 0.92745) < 5e-6
+    assert abs(rs.sample(rs.u_star - 1e-9)[0] - 0.42632) < 5e-6
+    assert abs(rs.sample(rs.u_star + 1e-9)[0] - 0.26557) < 5e-6
+    assert abs(rs.shock_speed_right() - 1.75216) < 5e-6
+    # undisturbed states outside the waves, continuity through the left fan
+    assert rs.sample(-2.0) == (1.0, 0.0, 1.0) and rs.sample(2.0) == (0.125, 0.0, 0.1)
+    head, tail = -math.sqrt(1.4), rs.u_star - math.sqrt(1.4) * (rs.p_star / 1.0) ** (0.4 / 2.8)
+    a, b = rs.sample(head + 1e-12), rs.sample(tail - 1e-12)
+    assert abs(a[0] - 1.0) < 1e-9 and abs(b[2] - rs.p_star) < 1e-9

@@ -28,9 +28,21 @@ def run(ic, which, steps, seed=12345):
     return states, dts, s.cell_info()
 
 
-@pytest.mark.parametrize("case,steps", [("sheared_cube", 3), ("tiny_impact", 4), ("config0", 2)])
+CASES = {
+    "sheared_cube": ics.sheared_cube,
+    "tiny_impact": ics.tiny_impact,
+    "config0": ics.config0,
+    # periodic y/z, ideal gas, Price conductivity (alpha_u = 1), frozen particles
+    "sod_small": lambda: ics.sod(40, 8),
+    # periodic y/z, basalt with strength, frozen ends
+    "elastic_wave": lambda: ics.elastic_wave(dx=0.02),
+}
+
+
+@pytest.mark.parametrize("case,steps", [("sheared_cube", 3), ("tiny_impact", 4), ("config0", 2),
+                                        ("sod_small", 4), ("elastic_wave", 3)])
 def test_bitwise_trajectory(case, steps):
-    ic = getattr(ics, case)()
+    ic = CASES[case]()
     a, da, ca = run(ic, "oracle", steps)
     b, db, cb = run(ic, "ref", steps)
     assert da == db
