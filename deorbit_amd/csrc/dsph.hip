@@ -83,6 +83,10 @@ class Engine : public EngineBase {
         tune_.cell_capacity_per_particle = 4;
         tune_.neighbor_capacity = 0;
         tune_.list_skin = 0.06;
+        for (const auto& p : segp_)
+            if ((p.periodic[0] && p.hi[0] > p.lo[0]) || (p.periodic[1] && p.hi[1] > p.lo[1]) ||
+                (p.periodic[2] && p.hi[2] > p.lo[2]))
+                periodic_ = true;
         for (auto& e : prof_total_) e = 0.0;
         for (auto& e : prof_count_) e = 0;
     }
@@ -438,8 +442,16 @@ class Engine : public EngineBase {
 
     // Up to four rounds of two iterations; rounds a segment does not need exit
     // at once on the device.  Round 0's first iteration builds the neighbour
-    // lists that its second iteration and the force pass read.
+    // lists that its second iteration and the force pass read.  Contexts with a
+    // periodic segment run the pair kernels compiled with periodic images.
     void density() {
+        if (periodic_) density_rounds<true>();
+        else density_rounds<false>();
+        launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, 4);
+        records_ = true;
+    }
+    template <bool PER>
+    void density_rounds() {
         const int bq = tune_.block_pair;
         const unsigned nb = blocks_for(n_, bq);
         const int lists = w_.nbr_cap > 0 ? 1 : 0;
@@ -447,23 +459,21 @@ class Engine : public EngineBase {
             launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, r);
             const auto th = hmax_target(r + 1);
             if (r == 0 && lists)
-                launch(PH_DENSITY_BUILD, "density_build", k_density_build<R>, dim3(nb), dim3(bq), cur(), w_,
+                launch(PH_DENSITY_BUILD, "density_build", k_density_build<R, PER>, dim3(nb), dim3(bq), cur(), w_,
                        tune_.xcd_remap);
             else
-                launch(PH_DENSITY, "density", k_density<R, false>, dim3(nb), dim3(bq), cur(), w_, r,
+                launch(PH_DENSITY, "density", k_density<R, false, PER>, dim3(nb), dim3(bq), cur(), w_, r,
                        tune_.xcd_remap, lists, th);
-            launch(PH_DENSITY, "density", k_density<R, true>, dim3(nb), dim3(bq), cur(), w_, r, tune_.xcd_remap,
-                   lists, th);
+            launch(PH_DENSITY, "density", k_density<R, true, PER>, dim3(nb), dim3(bq), cur(), w_, r,
+                   tune_.xcd_remap, lists, th);
             fold_parts(PH_DENSITY, th, nb);
             if (r == 0 && lists) {
-                launch(PH_DENSITY_LIST, "density_list", k_density_list<R>, dim3(nb), dim3(bq), cur(), w_,
+                launch(PH_DENSITY_LIST, "density_list", k_density_list<R, PER>, dim3(nb), dim3(bq), cur(), w_,
                        tune_.xcd_remap, th);
                 fold_parts(PH_DENSITY, th, nb);
             }
         }
         lists_built_ = lists != 0;
-        launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, 4);
-        records_ = true;
     }
 
     void finalize_rounds() {
@@ -471,18 +481,24 @@ class Engine : public EngineBase {
     }
 
     void force() {
+        if (periodic_) force_pass<true>();
+        else force_pass<false>();
+        have_dt_ = true;
+    }
+    template <bool PER>
+    void force_pass() {
         const int bq = tune_.block_pair;
         const auto td = dt_target();
         launch(PH_ROUND, "reset_dt", k_reset_dt, dim3(1), dim3(256), d_st_, nseg_);
         const unsigned nb = blocks_for(n_, bq);
         const int lists = lists_built_ ? 1 : 0;
-        launch(PH_FORCE, "force", k_force<R>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, lists, td);
+        launch(PH_FORCE, "force", k_force<R, PER>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, lists, td);
         if (lists) {
             fold_parts(PH_FORCE, td, nb);
-            launch(PH_FORCE_LIST, "force_list", k_force_list<R>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, td);
+            launch(PH_FORCE_LIST, "force_list", k_force_list<R, PER>, dim3(nb), dim3(bq), cur(), w_,
+                   tune_.xcd_remap, td);
         }
-        fold_parts(PH_FORCE, td, blocks_for(n_, bq));
-        have_dt_ = true;
+        fold_parts(PH_FORCE, td, nb);
     }
 
     void finish(int damage, int kick) {
@@ -1010,6 +1026,7 @@ class Engine : public EngineBase {
     dsph_tuning tune_{};
     bool built_ = false, records_ = false, have_dt_ = false;
     bool lists_built_ = false;     // the last density pass built neighbour lists
+    bool periodic_ = false;        // some segment has a periodic axis
     bool profiling_ = false, capturing_ = false;
     std::vector<Pending> pending_;
     std::vector<hipEvent_t> ev_pool_;

@@ -52,13 +52,13 @@ __device__ __forceinline__ void min_image(const SegGrid& g, R& rx, R& ry, R& rz)
 // corner of the visited image relative to the own cell's corner.  A periodic
 // axis with fewer cells than the stencil is walked once over all its cells
 // (offsets -(dim-1)/2 .. dim-1-(dim-1)/2) without culling.
-template <class R>
+template <class R, bool PER>
 struct AxisWalk {
     int c, dim, lo, hi, per;
     bool cull;
     R o, cw;
     __device__ __forceinline__ void init(const SegGrid& g, int a, int cc, R oo, double reach) {
-        c = cc; dim = g.dim[a]; per = g.per[a]; o = oo; cw = (R)g.cw[a];
+        c = cc; dim = g.dim[a]; per = PER && g.per[a]; o = oo; cw = (R)g.cw[a];
         int rad = (int)ceil(reach * g.inv_cw[a]);
         rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
         cull = true;
@@ -205,7 +205,7 @@ __device__ __forceinline__ uint32_t quad_get(const uint4& v, int s) {
 // FP32 path, own-cell offset shifted by the cell delta (so the pair separation
 // is formed from numbers below one cell width), taking on a periodic axis the
 // image the stencil walk uses; FP64 path, absolute position.
-template <class R>
+template <class R, bool PER>
 __device__ __forceinline__ void rel_base(const Work<R>& w, const SegGrid& g, uint32_t k, int cx, int cy, int cz,
                                          R ox, R oy, R oz, R ax, R ay, R az, R& bx, R& by, R& bz) {
     if (sizeof(R) == 4) {
@@ -214,7 +214,7 @@ __device__ __forceinline__ void rel_base(const Work<R>& w, const SegGrid& g, uin
         R sh[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            if (g.per[a]) {
+            if (PER && g.per[a]) {
                 const int m = g.dim[a], lo = -((m - 1) / 2), hi = m - 1 - (m - 1) / 2;
                 if (d[a] < lo) d[a] += m;
                 else if (d[a] > hi) d[a] -= m;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void rel_base(const Work<R>& w, const SegGrid& g, uin
 
 // ---------------------------------------------------------------- density
 // Per-particle constants of one density iteration.
-template <class R>
+template <class R, bool PER>
 struct DensLane {
     using V4 = typename Vec4<R>::T;
     V4 me;                 // own density record
@@ -243,7 +243,7 @@ struct DensLane {
     __device__ void setup(const Carried<R>& c, const SegPar& p, const SegGrid& g, int64_t i, uint32_t cc) {
         cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
         gp = &g;
-        per = g.per[0] || g.per[1] || g.per[2];
+        per = PER && (g.per[0] || g.per[1] || g.per[2]);
         two_h = R(2) * h;
         th2 = two_h * two_h;
         inv_h = R(1) / h;
@@ -260,7 +260,7 @@ struct DensLane {
     // FP64 tests r < 2h exactly as the reference does
     __device__ __forceinline__ R term(const V4& q, R bx, R by, R bz) const {
         R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
-        if (per) min_image(*gp, rx, ry, rz);
+        if (PER && per) min_image(*gp, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R rr = f_sqrt(r2);
         const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
@@ -279,11 +279,41 @@ struct DensLane {
 // Visit the sort cells whose nearest point is closer than `reach` to particle
 // i, in lexicographic cell order; fn(key, sx, sy, sz, gap^2) per cell, with
 // (sx, sy, sz) the frame shift of the visited cell (see AxisWalk).
-template <class R, class F>
+template <class R, bool PER, class F>
 __device__ __forceinline__ void for_cells(const SegGrid& g, int cx, int cy, int cz, R ox, R oy, R oz, R reach,
                                           F&& fn) {
     const R reach2 = reach * reach;
-    AxisWalk<R> X, Y, Z;
+    if constexpr (!PER) {
+        // no periodic axis: every axis has the same cell width, one radius
+        // serves all three and the walk state fits fewer registers
+        const R cw = (R)g.cw[0];
+        int rad = (int)ceil((double)reach * g.inv_cw[0]);
+        rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
+        for (int dx = -rad; dx <= rad; dx++) {
+            const int nx = cx + dx;
+            if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
+            const R gx = cell_gap(dx, ox, cw);
+            const R gx2 = gx * gx;
+            if (gx2 >= reach2) continue;
+            for (int dy = -rad; dy <= rad; dy++) {
+                const int ny = cy + dy;
+                if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
+                const R gy = cell_gap(dy, oy, cw);
+                const R gxy = gx2 + gy * gy;
+                if (gxy >= reach2) continue;
+                for (int dz = -rad; dz <= rad; dz++) {
+                    const int nz = cz + dz;
+                    if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
+                    const R gz = cell_gap(dz, oz, cw);
+                    const R gg = gxy + gz * gz;
+                    if (gg >= reach2) continue;
+                    fn(cell_key(g, nx, ny, nz), (R)dx * cw, (R)dy * cw, (R)dz * cw, gg);
+                }
+            }
+        }
+        return;
+    }
+    AxisWalk<R, PER> X, Y, Z;
     X.init(g, 0, cx, ox, (double)reach);
     Y.init(g, 1, cy, oy, (double)reach);
     Z.init(g, 2, cz, oz, (double)reach);
@@ -313,12 +343,12 @@ __device__ __forceinline__ void for_cells(const SegGrid& g, int cx, int cy, int 
 
 // Sum over the sort cells within 2h of particle i, in lexicographic cell order
 // and cell-table order inside a cell.
-template <class R>
+template <class R, bool PER>
 __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p, const SegGrid& g,
-                          const DensLane<R>& L) {
+                          const DensLane<R, PER>& L) {
     using V4 = typename Vec4<R>::T;
     R s = R(0);
-    for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.two_h, [&](uint32_t key, R sx, R sy, R sz, R) {
+    for_cells<R, PER>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.two_h, [&](uint32_t key, R sx, R sy, R sz, R) {
         const uint2 rg = w.table[key];
         R bx, by, bz;
         L.frame(sx, sy, sz, bx, by, bz);
@@ -379,7 +409,7 @@ __device__ __forceinline__ R density_store(const Carried<R>& c, const Work<R>& w
 // round test and runs the EOS epilogue.  In the second iteration of round 0
 // with lists built, lanes whose list is usable are left to k_density_list;
 // the others walk and mark their list unusable for the force pass too.
-template <class R, bool LAST>
+template <class R, bool LAST, bool PER>
 __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r, int remap, int lists,
                                                  SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -395,7 +425,7 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
-        DensLane<R> L;
+        DensLane<R, PER> L;
         L.h = c.h[i];
         if (by_list && dens_list_usable<R>(w, i, L.h)) {
             act = false;                           // k_density_list owns this particle
@@ -408,7 +438,7 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
             R rho = rho_floor<R>();
             if (finite) {
                 L.setup(c, p, g, i, cc);
-                rho = smax(density_walk<R>(c, w, p, g, L), rho_floor<R>());
+                rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
             }
             hnew_d = (double)density_store<R, LAST>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
         }
@@ -422,7 +452,7 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
 // Second density iteration of round 0 for the particles whose list is usable:
 // the list holds the walk's pairs in the walk's order, so the sum is the one
 // the walk would give.  Four entries are read ahead so their loads overlap.
-template <class R>
+template <class R, bool PER>
 __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, int remap, SegTargets<1> tg) {
     using V4 = typename Vec4<R>::T;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -437,7 +467,7 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
-        DensLane<R> L;
+        DensLane<R, PER> L;
         L.h = c.h[i];
         const uint32_t cc = w.cellc[i];
         L.me = w.dens_rec[i];
@@ -449,7 +479,7 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
         auto add = [&](uint32_t k, const V4& q) {
             if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) return;
             R bx, by, bz;
-            rel_base<R>(w, g, k, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.me.x, L.me.y, L.me.z, bx, by, bz);
+            rel_base<R, PER>(w, g, k, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.me.x, L.me.y, L.me.z, bx, by, bz);
             s += L.term(q, bx, by, bz);
         };
         for (uint32_t e = 0; e < nc; e += 4) {
@@ -473,7 +503,7 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
 // enters i's list in walk order (hcell_j: bound of h over j's sort cell at
 // the start of the step).  That covers r < 2h_i and r < h_i + h_j for as long
 // as no h grows by more than the skin.  Lists that do not fit are marked.
-template <class R>
+template <class R, bool PER>
 __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, int remap) {
     using V4 = typename Vec4<R>::T;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -484,7 +514,7 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
     if (w.st[seg].active[0]) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
-        DensLane<R> L;
+        DensLane<R, PER> L;
         L.h = c.h[i];
         const uint32_t cc = w.cellc[i];
         L.me = w.dens_rec[i];
@@ -501,7 +531,7 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
             // entries are gathered four at a time and stored as one 16-byte quad;
             // unused lanes of a last partial quad keep valid indices
             uint4 quad = make_uint4((uint32_t)i, (uint32_t)i, (uint32_t)i, (uint32_t)i);
-            for_cells<R>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, sk * (L.h + smax(L.h, hm)),
+            for_cells<R, PER>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, sk * (L.h + smax(L.h, hm)),
                          [&](uint32_t key, R sx, R sy, R sz, R gg) {
                 const R rb = sk * (L.h + smax(L.h, (R)w.cell_hmax[key]));
                 const R rb2 = rb * rb;
@@ -511,7 +541,7 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
                 L.frame(sx, sy, sz, bx, by, bz);
                 auto visit = [&](uint32_t k, const V4& q) {
                     R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
-                    if (L.per) min_image(g, rx, ry, rz);
+                    if (PER && L.per) min_image(g, rx, ry, rz);
                     const R r2 = rx * rx + ry * ry + rz * rz;
                     if (r2 < rb2) {
                         if (cnt < (uint32_t)cap) {
@@ -571,7 +601,7 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
 //   a     -= m_j (term + (R_i + R_j)(W/W(hbar/eta))^4) grad W
 //   a     += m_j (S_i d_i/rho_i^2 + S_j d_j/rho_j^2) . grad W
 // then the Jaumann stress rate, the deviatoric work and the CFL step.
-template <class R>
+template <class R, bool PER>
 struct ForceLane {
     using V4 = typename Vec4<R>::T;
     V4 pi, vi, sai, sbi;
@@ -596,7 +626,7 @@ struct ForceLane {
         inv_fdp = (eps_as > R(0)) ? R(1) / kern_shape(R(1) / (R)p.eta) : R(0);
         cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
         gp = &g;
-        per = g.per[0] || g.per[1] || g.per[2];
+        per = PER && (g.per[0] || g.per[1] || g.per[2]);
         ui = alpha_u > R(0) ? (R)c.u[i] : R(0);
         Pi = alpha_u > R(0) ? w.P[i] : R(0);
     }
@@ -655,7 +685,7 @@ struct ForceLane {
     __device__ __forceinline__ bool in_range(const V4& q, R bx, R by, R bz, R& rx, R& ry, R& rz, R& r2,
                                              R& th) const {
         rx = bx - q.x; ry = by - q.y; rz = bz - q.z;
-        if (per) min_image(*gp, rx, ry, rz);
+        if (PER && per) min_image(*gp, rx, ry, rz);
         r2 = rx * rx + ry * ry + rz * rz;
         th = hi + q.w;
         return r2 < th * th && r2 != R(0);
@@ -717,7 +747,7 @@ __device__ __forceinline__ bool force_list_usable(const Work<R>& w, int64_t i, i
 // Force pass by walking the cells (radius h_i + max h, culled with the per-cell
 // bound of h).  With lists built, particles whose list is usable are left to
 // k_force_list.
-template <class R>
+template <class R, bool PER>
 __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, int lists, SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
@@ -732,11 +762,11 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
             const SegPar& p = w.par[seg];
             const SegGrid& g = w.grid[seg];
             const uint32_t cc = w.cellc[i];
-            ForceLane<R> F;
+            ForceLane<R, PER> F;
             F.setup(c, w, p, g, i, cc);
             own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
             if (isfinite((double)F.ox) && isfinite((double)F.oy) && isfinite((double)F.oz)) {
-                for_cells<R>(g, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.hi + (R)w.st[seg].hmax_final,
+                for_cells<R, PER>(g, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.hi + (R)w.st[seg].hmax_final,
                              [&](uint32_t key, R sx, R sy, R sz, R gg) {
                     const R reach = F.hi + (R)w.cell_hmax[key];
                     if (gg >= reach * reach) return;
@@ -773,7 +803,7 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
 // Force pass over the neighbour lists.  List entries are mostly pairs in
 // range, so every record of two entries is read up front and their loads
 // overlap.
-template <class R>
+template <class R, bool PER>
 __global__ __launch_bounds__(256) void k_force_list(Carried<R> c, Work<R> w, int remap, SegTargets<1> tg) {
     using V4 = typename Vec4<R>::T;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -789,7 +819,7 @@ __global__ __launch_bounds__(256) void k_force_list(Carried<R> c, Work<R> w, int
             const SegPar& p = w.par[seg];
             const SegGrid& g = w.grid[seg];
             const uint32_t cc = w.cellc[i];
-            ForceLane<R> F;
+            ForceLane<R, PER> F;
             F.setup(c, w, p, g, i, cc);
             own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
             const uint32_t nc = w.nbr_cnt[i];
@@ -802,9 +832,9 @@ __global__ __launch_bounds__(256) void k_force_list(Carried<R> c, Work<R> w, int
                 const V4 a0 = w.frec_sa[k0], a1 = w.frec_sa[k1];
                 const V4 b0 = w.frec_sb[k0], b1 = w.frec_sb[k1];
                 R bx, by, bz, rx, ry, rz, r2, th;
-                rel_base<R>(w, *F.gp, k0, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
+                rel_base<R, PER>(w, *F.gp, k0, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
                 if (F.in_range(q0, bx, by, bz, rx, ry, rz, r2, th)) F.body(c, w, k0, rx, ry, rz, r2, th, v0, t0, a0, b0);
-                rel_base<R>(w, *F.gp, k1, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
+                rel_base<R, PER>(w, *F.gp, k1, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.pi.x, F.pi.y, F.pi.z, bx, by, bz);
                 if (has1 && F.in_range(q1, bx, by, bz, rx, ry, rz, r2, th))
                     F.body(c, w, k1, rx, ry, rz, r2, th, v1, t1, a1, b1);
             };
