@@ -1,6 +1,6 @@
 """Throughput of the MI355X SPH impact step (one JSON line on stdout).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload graze_1m] [--no-cpu]
 
 A step is one System::step of every particle of the workload (kick/drift, cell
 sort, adaptive-h density, EOS, strain + forces, damage, finish), with the CFL
@@ -32,14 +32,17 @@ sys.path.insert(0, ROOT)
 
 from deorbit_amd import dsph, ics, roofline  # noqa: E402
 
-CONFIGS = {0: ics.config0, 1: ics.config1, 2: ics.config2, 3: None, 4: ics.config4}
+WORKLOADS = {"impact_37k": ics.impact_37k, "graze_1m": ics.graze_1m, "layered_4m": ics.layered_4m,
+             "ensemble_256": None, "halfspace_32m": ics.halfspace_32m}
 WORKLOAD = {
-    0: "config0: R=10dx iron sphere at 45 deg, 8 km/s, onto a 64x32x16 granite slab, no strength",
-    1: "config1: R=40dx iron sphere at 3 deg, 8 km/s, into a 128x64x96 granite slab, strength+damage",
-    2: "config2: R=60dx iron sphere at 2 deg, 8 km/s, into 192x96 columns of 32 water over 136 granite",
-    3: "config3: 256 independent iron-on-granite impacts in one batch (N logU[8k,200k], 1-45 deg, "
-       "4-12 km/s, impactor/target 0.05-0.3, strength+damage), 500 steps per instance",
-    4: "config4: R=120dx iron sphere at 1.5 deg, 8 km/s, onto a 384x192x339 granite half-space",
+    "impact_37k": "impact_37k: R=10dx iron sphere at 45 deg, 8 km/s, onto a 64x32x16 granite slab, no strength",
+    "graze_1m": "graze_1m: R=40dx iron sphere at 3 deg, 8 km/s, into a 128x64x96 granite slab, strength+damage",
+    "layered_4m": "layered_4m: R=60dx iron sphere at 2 deg, 8 km/s, into 192x96 columns of 32 water over 136 "
+                  "granite",
+    "ensemble_256": "ensemble_256: 256 independent iron-on-granite impacts in one batch (N logU[8k,200k], "
+                    "1-45 deg, 4-12 km/s, impactor/target 0.05-0.3, strength+damage), 500 steps per instance",
+    "halfspace_32m": "halfspace_32m: R=120dx iron sphere at 1.5 deg, 8 km/s, onto a 384x192x339 granite "
+                     "half-space",
 }
 ENSEMBLE_STEPS = 500       # steps per instance of the batched ensemble
 
@@ -108,7 +111,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(CONFIGS))
+    ap.add_argument("--workload", default="graze_1m", choices=list(WORKLOADS))
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -132,13 +135,13 @@ def main():
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.config == 3:
+    if args.workload == "ensemble_256":
         ic = ics.ensemble(256, ENSEMBLE_STEPS)
         n = sum(len(m["mass"]) for m in ic)
         log(f"workload ensemble of {len(ic)} impacts: n={n}")
         cpu_ic = sorted(ic, key=lambda m: len(m["mass"]))[len(ic) // 2]     # the median instance
     else:
-        ic = CONFIGS[args.config]()
+        ic = WORKLOADS[args.workload]()
         n = len(ic["mass"])
         log(f"workload {ic['meta']['name']}: n={n}")
         cpu_ic = ics.impact_region(ic, 1_100_000)     # the CPU solver's share of a large impact
@@ -155,12 +158,13 @@ def main():
     # dominant kernel: the force pass, over neighbour lists or by cell walks
     # (whichever the tuning selected)
     phase = "force_list" if prof.get("force_list", (0.0, 0))[0] > prof["force"][0] else "force"
-    kname = "k_force_list<float>" if phase == "force_list" else "k_force<float>"
+    # (the benchmark workloads have no periodic axis: the non-periodic kernel variant)
+    kname = "k_force_list<float, false>" if phase == "force_list" else "k_force<float, false>"
     force_ms = prof[phase][0] / max(1, prof[phase][1])
     achieved = kb["force"] * n / (force_ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile of this
     # kernel and workload, used only when it was recorded for these device sources
-    rec = roofline.recorded_traffic(os.path.join(ROOT, "profiles"), kname, f"config{args.config}")
+    rec = roofline.recorded_traffic(os.path.join(ROOT, "profiles"), kname, args.workload)
     roof = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
             "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5),
@@ -184,7 +188,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(cpu_ic)
         if cpu_ic is not ic:
-            what = "median-size ensemble instance" if args.config == 3 else "impact region"
+            what = "median-size ensemble instance" if args.workload == "ensemble_256" else "impact region"
             cpu["sample"] = f"{what} ({len(cpu_ic['mass'])} particles): " + cpu["sample"]
         log(f"cpu: {cpu['value'] / 1e6:.3f} M particle-steps/s ({cpu['kind']}, {cpu['cores']} threads)")
 
@@ -194,11 +198,11 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 pair/EOS/stress, f64 state+reductions",
         "data": "synthetic (seeded lattice impact IC)",
-        "config": {"workload": WORKLOAD[args.config], "particles": n, "dt": "device CFL (cfl=0.2)",
+        "config": {"workload": WORKLOAD[args.workload], "particles": n, "dt": "device CFL (cfl=0.2)",
                    "graph": True, "t_sim_end": diag["t"]},
         "roofline": roof, "cpu_baseline": cpu, "fp64": fp64,
     }
-    if args.config == 3:
+    if args.workload == "ensemble_256":
         # every batched step advances all 256 instances by one step
         line["runs_per_hour"] = len(ic) * 3600.0 / (ENSEMBLE_STEPS * el / args.steps)
         line["config"]["instances"] = len(ic)

@@ -112,26 +112,26 @@ def impact(R: int, theta_deg: float, speed: float, nx: int, ny: int, layers, dx:
 
 
 # ------------------------------------------------------------------ workloads
-def config0():
+def impact_37k():
     """Plain FP64 CPU-runnable case: R=10dx iron sphere at 45 deg, 8 km/s, onto a
     64x32x16 granite slab; no strength; 100 steps."""
-    return impact(10, 45.0, 8000.0, 64, 32, [(M.MAT_TARGET, 16)], seed=1000, name="config0",
+    return impact(10, 45.0, 8000.0, 64, 32, [(M.MAT_TARGET, 16)], seed=1000, name="impact_37k",
                   steps=100)
 
 
-def config1():
+def graze_1m():
     """~1.05M particles: R=40dx at 3 deg, 8 km/s, into a 128x64x96 granite slab;
     strength (iron Y=1 GPa, G=80 GPa; granite Y=2 GPa, G=30 GPa) and Weibull damage
     (k=1e39 m^-3, m=9) on; 500 steps."""
     return impact(40, 3.0, 8000.0, 128, 64, [(M.MAT_TARGET, 96)], seed=1001, strength=True,
-                  frac=True, name="config1", steps=500)
+                  frac=True, name="graze_1m", steps=500)
 
 
-def config2():
+def layered_4m():
     """~4.0M particles: R=60dx at 2 deg, 8 km/s, into 192x96 columns of 32 water
     cells over 136 granite cells; strength and damage on; 1000 steps."""
     return impact(60, 2.0, 8000.0, 192, 96, [(M.MAT_WATER, 32), (M.MAT_TARGET, 136)],
-                  seed=1002, strength=True, frac=True, name="config2", steps=1000)
+                  seed=1002, strength=True, frac=True, name="layered_4m", steps=1000)
 
 
 def gaussian_relief(nx, ny, amplitude, corr, seed):
@@ -148,13 +148,13 @@ def gaussian_relief(nx, ny, amplitude, corr, seed):
     return f * (amplitude / sd if sd > 0 else 0.0)
 
 
-def config4():
+def halfspace_32m():
     """~32M particles: R=120dx at 1.5 deg, 8 km/s, onto a 384x192 granite half-space
     ~339 cells deep with a 16-cell top layer at 0.98 rho0 (mass-scaled) and a
     Gaussian surface relief (4dx amplitude, 30dx correlation, seed 4000); 200 steps."""
     rel = gaussian_relief(384, 192, 4.0, 30.0, 4000)
     return impact(120, 1.5, 8000.0, 384, 192, [(M.MAT_TARGET, 339)], seed=4001, strength=True,
-                  frac=True, relief=rel, porous_top=(16, 0.98), name="config4", steps=200)
+                  frac=True, relief=rel, porous_top=(16, 0.98), name="halfspace_32m", steps=200)
 
 
 def ensemble_members(count: int = 256, seed0: int = 3000, nmin: int = 8000, nmax: int = 200000):

@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
-    ap.add_argument("--workload", default="config1")
+    ap.add_argument("--workload", default="graze_1m")
     ap.add_argument("--bench-json")
     a = ap.parse_args()
     if a.what == "traffic":

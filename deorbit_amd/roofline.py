@@ -72,13 +72,15 @@ def recorded_traffic(folder: str, kernel: str, workload: str) -> dict | None:
     digest; None when absent or recorded for other sources."""
     import glob
     import os
+    import re
     digest = source_digest()
     paths = sorted(glob.glob(os.path.join(folder, "*.md")) + glob.glob(os.path.join(folder, "*.txt")))
     for path in paths:
         with open(path) as f:
             lines = [ln for ln in f if ln.startswith("traffic:")]
         for line in lines:
-            kv = dict(tok.split("=", 1) for tok in line[len("traffic:"):].split())
+            # values run to the next " key=" (kernel names hold spaces)
+            kv = dict(re.findall(r"(\w+)=(.*?)(?=\s+\w+=|\s*$)", line[len("traffic:"):].strip()))
             if kv.get("kernel") == kernel and kv.get("workload") == workload and kv.get("digest") == digest:
                 return {"bytes": float(kv["bytes"]), "fetch_factor": float(kv["fetch_factor"]),
                         "write_factor": float(kv["write_factor"])}

@@ -12,7 +12,7 @@ from the reference tree into oracle/_ref/ by `make -C oracle ref`:
 * eig_yield.npz   -- eig_max (sph.hpp:41-54) and yield_limit (sph.hpp:320-331)
                      on random symmetric tensors.
 * weibull.npz     -- seed_damage (sph.hpp:117-128) flaw strains.
-* config0_ref.npz -- the CPU-runnable impact workload: dt sequence, energy and
+* impact_37k_ref.npz -- the CPU-runnable impact workload: dt sequence, energy and
                      aggregates for 100 steps, every 4th particle after 1 step and
                      every 16th particle after 10 and 100 steps.
 * tiny_ref.npz    -- a 1,217-particle impact with strength and damage: full
@@ -96,7 +96,7 @@ def eig_yield():
 
 
 def weibull():
-    ic = ics.config1()
+    ic = ics.graze_1m()
     out = {}
     for seed in (12345, 1, 4242):
         s = po.make_system(ic, ic["params"], ic["materials"], "ref")
@@ -124,8 +124,8 @@ def energy(st):
     return float(np.sum(0.5 * st["mass"] * np.sum(v * v, axis=1)) + np.sum(st["mass"] * st["u"]))
 
 
-def config0():
-    ic = ics.config0()
+def impact_37k():
+    ic = ics.impact_37k()
     s = po.make_system(ic, ic["params"], ic["materials"], "ref")
     s.init()
     dts, E, agg, mom = [], [], [], []
@@ -146,8 +146,8 @@ def config0():
         if k + 1 in (10, 100):
             for f in ("pos", "vel", "u", "rho", "hh", "P"):
                 keep[f"s{k + 1}_{f}"] = st[f][::16].copy()
-        print("config0 step", k + 1, flush=True)
-    np.savez_compressed(os.path.join(HERE, "config0_ref.npz"), dt=np.array(dts), E=np.array(E),
+        print("impact_37k step", k + 1, flush=True)
+    np.savez_compressed(os.path.join(HERE, "impact_37k_ref.npz"), dt=np.array(dts), E=np.array(E),
                         agg=np.array(agg), mom=np.array(mom), **keep)
 
 
@@ -175,6 +175,6 @@ def tiny():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["force_ref", "eos_sweep", "eig_yield", "weibull", "tiny", "config0"]
+    which = sys.argv[1:] or ["force_ref", "eos_sweep", "eig_yield", "weibull", "tiny", "impact_37k"]
     for w in which:
         globals()[w]()

@@ -7,8 +7,8 @@ import pytest
 from deorbit_amd import formats, ics
 
 
-def test_config0_particle_counts_and_geometry():
-    ic = ics.config0()
+def test_impact_37k_particle_counts_and_geometry():
+    ic = ics.impact_37k()
     assert ic["meta"]["n_target"] == 64 * 32 * 16
     assert ic["meta"]["n_impactor"] == 4169            # lattice points with i^2+j^2+k^2 <= 100
     assert len(ic["mass"]) == 36937
@@ -24,7 +24,7 @@ def test_config0_particle_counts_and_geometry():
 
 
 def test_generators_are_deterministic():
-    a, b = ics.config0(), ics.config0()
+    a, b = ics.impact_37k(), ics.impact_37k()
     for k in ("pos", "vel", "mass", "mat"):
         assert np.array_equal(a[k], b[k])
     c = ics.impact(10, 45.0, 8000.0, 64, 32, [(0, 16)], seed=1001)
@@ -149,10 +149,24 @@ def test_ensemble_batch_and_impact_region_sample():
     E = ics.ensemble(8, 500)
     assert len(E) == 8 and all(e["meta"]["steps"] == 500 for e in E)
     assert all(8000 * 0.8 < len(e["mass"]) < 200000 * 1.2 for e in E)
-    ic = ics.config1()
+    ic = ics.graze_1m()
     r = ics.impact_region(ic, 200000)
     assert len(r["mass"]) == 200000
     imp = int((r["mat"] == 1).sum())
     assert 0.2 * 200000 < imp < 0.8 * 200000          # both impactor and target in the sample
     assert ics.impact_region(ic, 10 ** 7) is ic
 
+
+
+def test_recorded_traffic_lines_are_matched_by_kernel_workload_and_digest(tmp_path):
+    from deorbit_amd import roofline
+    d = roofline.source_digest()
+    (tmp_path / "a.md").write_text(
+        "text\n"
+        f"traffic: kernel=k_force<float, false> workload=graze_1m digest={d} bytes=3.8e+08 "
+        "fetch_factor=1.9999 write_factor=1.0000\n"
+        "traffic: kernel=k_force<float, false> workload=graze_1m digest=0000 bytes=1 fetch_factor=1 write_factor=1\n")
+    rec = roofline.recorded_traffic(str(tmp_path), "k_force<float, false>", "graze_1m")
+    assert rec == {"bytes": 3.8e8, "fetch_factor": 1.9999, "write_factor": 1.0}
+    assert roofline.recorded_traffic(str(tmp_path), "k_force<float, true>", "graze_1m") is None
+    assert roofline.recorded_traffic(str(tmp_path), "k_force<float, false>", "layered_4m") is None

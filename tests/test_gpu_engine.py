@@ -59,7 +59,7 @@ def test_neighbour_lists_give_the_cell_walk_results(precision, gpu):
     # multiply-adds differently in the list and walk kernels: round-off of the
     # working precision (scale-normalised 1e-13 in FP64, 1e-5 in FP32 after 12
     # steps).
-    ic = ics.config0()
+    ic = ics.impact_37k()
     steps = 12
     ref, dref = run_states(ic, precision, steps, True, dict(neighbor_capacity=-1))
     assert dref["list_walk_steps"] == 0 and dref["list_overflow"] == 0
@@ -153,10 +153,10 @@ def test_batch_of_segments_equals_separate_runs(precision, gpu):
     assert len(dts) == 3 and len(set(dts.tolist())) == 3       # each segment its own CFL step
 
 
-def test_full_size_invariants_config1(gpu):
+def test_full_size_invariants_graze_1m(gpu):
     """~1.05M particles with strength and damage: after init and after 10 steps the
     pairwise momentum exchange cancels (sum m a = 0) and h stays in its clamp."""
-    ic = ics.config1()
+    ic = ics.graze_1m()
     for precision in (dsph.MIXED, dsph.FP64):
         s = dsph.make_system(ic, precision)
         s.init()
@@ -174,8 +174,8 @@ def test_full_size_invariants_config1(gpu):
         s.close()
 
 
-def test_full_size_fp64_init_matches_oracle_config1(gpu):
-    ic = ics.config1()
+def test_full_size_fp64_init_matches_oracle_graze_1m(gpu):
+    ic = ics.graze_1m()
     o = po.make_system(ic, ic["params"], ic["materials"], seed_damage=12345, dx=ic["dx"])
     o.init()
     s = dsph.make_system(ic, dsph.FP64)
@@ -183,14 +183,14 @@ def test_full_size_fp64_init_matches_oracle_config1(gpu):
     for name, f in (("rho", po.RHO), ("hh", po.HH), ("P", po.P), ("acc", po.ACC), ("dudt", po.DUDT),
                     ("dSdt", po.DSDT), ("sigmax", po.SIGMAX)):
         e = scale_err(s.get(name), o.get(f))
-        print(f"config1 init fp64 {name}={e:.2e}", flush=True)
+        print(f"graze_1m init fp64 {name}={e:.2e}", flush=True)
         assert e < 1e-12, name
     assert abs(s.timestep() - o.timestep()) <= 1e-14 * o.timestep()
 
 
-def test_energy_rate_balance_config0_mixed(gpu):
+def test_energy_rate_balance_impact_37k_mixed(gpu):
     # no strength, no artificial stress: sum m (v.a + du/dt) vanishes pairwise
-    s = dsph.make_system(ics.config0(), dsph.MIXED)
+    s = dsph.make_system(ics.impact_37k(), dsph.MIXED)
     s.init()
     m, v, a, du = s.get("mass"), s.get("vel"), s.get("acc"), s.get("dudt")
     kin = m * np.sum(v * a, axis=1)

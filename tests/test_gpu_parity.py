@@ -108,11 +108,11 @@ def test_device_cfl_step_matches_oracle(precision, gpu):
     assert abs(d["t"] - g["dt"][:5].sum()) <= 10 * TOL[precision] * g["dt"][:5].sum()
 
 
-def test_config0_hundred_steps(gpu):
+def test_impact_37k_hundred_steps(gpu):
     """The CPU-runnable workload (36,937 particles) for 100 lockstep steps in both
     paths: fields after 1/10/100 steps and the energy drift against the fixture."""
-    g = golden("config0_ref.npz")
-    ic = ics.config0()
+    g = golden("impact_37k_ref.npz")
+    ic = ics.impact_37k()
     E0 = g["E"][0]
     drift_ref = (g["E"][-1] - E0) / E0
     for precision in PRECS:
@@ -125,13 +125,13 @@ def test_config0_hundred_steps(gpu):
             if k + 1 in (1, 10, 100):
                 sub = 4 if k == 0 else 16
                 errs = {f: scale_err(s.get(f)[::sub], g[f"s{k + 1}_{f}"]) for f in ("pos", "vel", "u", "rho", "hh", "P")}
-                report(f"config0 step {k + 1} {NAMES[precision]}", errs)
+                report(f"impact_37k step {k + 1} {NAMES[precision]}", errs)
                 lim = TOL[precision] * {1: 1, 10: 10, 100: 1e3}[k + 1]
                 for f, e in errs.items():
                     assert e < lim, (k + 1, f)
         d = s.diagnostics()
         drift = (d["e_tot"] - E0) / E0
-        print(f"config0 drift {NAMES[precision]} gpu={drift:.3e} oracle={drift_ref:.3e}", flush=True)
+        print(f"impact_37k drift {NAMES[precision]} gpu={drift:.3e} oracle={drift_ref:.3e}", flush=True)
         assert abs(drift - drift_ref) <= 1e-6
         mom = np.array(d["mom"])
         assert np.all(np.abs(mom - g["mom"][-1]) <= 1e-6 * np.abs(g["mom"][-1]).max())

@@ -92,7 +92,7 @@ def test_eig_max_and_yield_bitwise():
 
 def test_weibull_flaws_bitwise():
     g = golden("weibull.npz")
-    ic = ics.config1()
+    ic = ics.graze_1m()
     stride = int(g["stride"])
     assert np.array_equal(ic["mat"][::stride], g["mat"])
     for seed in (12345, 1, 4242):
@@ -121,9 +121,9 @@ def test_tiny_impact_trajectory_bitwise():
                 assert np.array_equal(st[f], g[f"s{k + 1}_{f}"]), (k + 1, f)
 
 
-def test_config0_trajectory_bitwise():
-    g = golden("config0_ref.npz")
-    ic = ics.config0()
+def test_impact_37k_trajectory_bitwise():
+    g = golden("impact_37k_ref.npz")
+    ic = ics.impact_37k()
     s = po.make_system(ic, ic["params"], ic["materials"], "oracle")
     s.init()
     for k in range(100):

@@ -165,7 +165,7 @@ def _state_after_init(ic, damage_seed=None):
 
 def test_momentum_conserved_pairwise():
     # antisymmetric grad W with symmetric pair terms: sum_i m_i a_i = 0
-    for ic, seed in ((ics.config0(), None), (ics.tiny_impact(), 12345)):
+    for ic, seed in ((ics.impact_37k(), None), (ics.tiny_impact(), 12345)):
         _, st = _state_after_init(ic, seed)
         ma = st["mass"][:, None] * st["acc"]
         assert np.all(np.abs(ma.sum(0)) <= 1e-11 * np.abs(ma).sum(0))
@@ -173,7 +173,7 @@ def test_momentum_conserved_pairwise():
 
 def test_energy_rate_balances_without_strength():
     # no strength, no artificial stress or conductivity: sum m (v.a + du/dt) = 0
-    _, st = _state_after_init(ics.config0())
+    _, st = _state_after_init(ics.impact_37k())
     m = st["mass"]
     kin = m * np.sum(st["vel"] * st["acc"], axis=1)
     th = m * st["dudt"]

@@ -31,7 +31,7 @@ def run(ic, which, steps, seed=12345):
 CASES = {
     "sheared_cube": ics.sheared_cube,
     "tiny_impact": ics.tiny_impact,
-    "config0": ics.config0,
+    "impact_37k": ics.impact_37k,
     # periodic y/z, ideal gas, Price conductivity (alpha_u = 1), frozen particles
     "sod_small": lambda: ics.sod(40, 8),
     # periodic y/z, basalt with strength, frozen ends
@@ -39,7 +39,7 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("case,steps", [("sheared_cube", 3), ("tiny_impact", 4), ("config0", 2),
+@pytest.mark.parametrize("case,steps", [("sheared_cube", 3), ("tiny_impact", 4), ("impact_37k", 2),
                                         ("sod_small", 4), ("elastic_wave", 3)])
 def test_bitwise_trajectory(case, steps):
     ic = CASES[case]()
