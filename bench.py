@@ -80,13 +80,37 @@ def profile_leg(s, steps):
     return prof
 
 
-def cpu_baseline(ic, seconds_budget=25.0):
-    """Reference solver on the host: init, then as many steps as fit the budget (>= 1)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def host_cores() -> dict:
+    """CPU model, logical CPUs, the CPUs this process may run on and its cgroup
+    CPU quota: the host threads a CPU solver can actually use here."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    if quota is not None:
+        usable = max(1, min(usable, int(quota)))
+    return {"cpu_model": model, "nproc": nproc, "usable_cores": usable, "cgroup_cpu_quota": quota}
+
+
+def time_cpu_solver(ic, which, nthreads, seconds_budget):
+    """Init, then as many steps as fit the budget (1 to 20); median step time."""
     from oracle import pyoracle as po
-    kind = "reference" if os.path.exists(po.REF_SO) else "port"
-    which = "ref" if kind == "reference" else "oracle"
-    o = po.make_system(ic, ic["params"], ic["materials"], which,
+    o = po.make_system(ic, ic["params"], ic["materials"], which, nthreads=nthreads,
                        seed_damage=12345 if ic["params"]["damage_on"] else None, dx=ic["dx"])
     t0 = time.perf_counter()
     o.init()
@@ -99,11 +123,33 @@ def cpu_baseline(ic, seconds_budget=25.0):
         times.append(time.perf_counter() - t0)
         if sum(times) + t_init > seconds_budget or len(times) >= 20:
             break
-    med = float(np.median(times))
-    cores = min(16, os.cpu_count() or 1)     # parallel_for caps at 16 threads (sph.hpp:58-60)
-    return {"value": len(ic["mass"]) / med, "unit": "particle-steps/s", "cores": cores, "kind": kind,
-            "sample": f"{len(times)} step(s) of the same workload after init, median {med * 1e3:.0f} ms/step "
-                      f"(init {t_init:.1f} s excluded); std::thread parallel_for over {cores} host threads"}
+    return float(np.median(times)), len(times), t_init
+
+
+def cpu_baseline(ic, seconds_budget=25.0):
+    """The reference solver on the host with its own thread cap (min(16, hardware
+    threads), sph.hpp:58-60); when this process may use more cores than that, the
+    oracle restatement (bit-identical to the reference) with the cap lifted too."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import pyoracle as po
+    host = host_cores()
+    kind = "reference" if os.path.exists(po.REF_SO) else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    cores = min(16, host["nproc"])
+    med, nsteps, t_init = time_cpu_solver(ic, which, cores, seconds_budget)
+    out = {"value": len(ic["mass"]) / med, "unit": "particle-steps/s", "cores": cores, "kind": kind,
+           "sample": f"{nsteps} step(s) of the same workload after init, median {med * 1e3:.0f} ms/step "
+                     f"(init {t_init:.1f} s excluded); std::thread parallel_for over {cores} host threads",
+           "host": host}
+    if host["usable_cores"] > cores:
+        med2, n2, _ = time_cpu_solver(ic, "oracle", host["usable_cores"], seconds_budget)
+        out["all_cores"] = {"value": len(ic["mass"]) / med2, "cores": host["usable_cores"], "kind": "port",
+                            "sample": f"{n2} step(s), median {med2 * 1e3:.0f} ms/step"}
+    else:
+        out["all_cores"] = {"value": out["value"], "cores": cores, "kind": kind,
+                            "sample": f"this process may use {host['usable_cores']} cores, no more than the "
+                                      "reference's own cap: the capped figure is the all-cores figure"}
+    return out
 
 
 def main():
@@ -190,7 +236,9 @@ def main():
         if cpu_ic is not ic:
             what = "median-size ensemble instance" if args.workload == "ensemble_256" else "impact region"
             cpu["sample"] = f"{what} ({len(cpu_ic['mass'])} particles): " + cpu["sample"]
-        log(f"cpu: {cpu['value'] / 1e6:.3f} M particle-steps/s ({cpu['kind']}, {cpu['cores']} threads)")
+        log(f"cpu: {cpu['value'] / 1e6:.3f} M particle-steps/s ({cpu['kind']}, {cpu['cores']} threads); "
+            f"all {cpu['all_cores']['cores']} usable cores: {cpu['all_cores']['value'] / 1e6:.3f} M/s; "
+            f"{cpu['host']['cpu_model']}, nproc {cpu['host']['nproc']}")
 
     line = {
         "metric": "SPH particle-steps/s on 1 MI355X, mixed vs FP64 at stated error; runs/hr",

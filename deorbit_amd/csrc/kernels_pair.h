@@ -95,14 +95,18 @@ __device__ __forceinline__ double f_sqrt(double x) { return sqrt(x); }
 //   f(q)  = t^3/4 - s^3          (1 - 1.5q^2 + 0.75q^3 for q < 1, (2-q)^3/4 for 1 <= q < 2)
 //   f'(q) = 3 s^2 - 0.75 t^2     (-3q + 2.25q^2 for q < 1, -0.75(2-q)^2 for 1 <= q < 2)
 // with t = max(2 - q, 0), s = max(1 - q, 0); W = f/(pi h^3), dW/dr = f'/(pi h^4).
+// (q is finite wherever the result is used, so the clamps are plain max
+// instructions; the NaN-propagating smax stays in the EOS and h updates)
+__device__ __forceinline__ float ramp(float x) { return fmaxf(x, 0.0f); }
+__device__ __forceinline__ double ramp(double x) { return fmax(x, 0.0); }
 template <class R>
 __device__ __forceinline__ R m4_shape(R q) {
-    const R t = smax(R(2) - q, R(0)), s = smax(R(1) - q, R(0));
+    const R t = ramp(R(2) - q), s = ramp(R(1) - q);
     return R(0.25) * t * t * t - s * s * s;
 }
 template <class R>
 __device__ __forceinline__ R m4_slope(R q) {
-    const R t = smax(R(2) - q, R(0)), s = smax(R(1) - q, R(0));
+    const R t = ramp(R(2) - q), s = ramp(R(1) - q);
     return R(3) * s * s - R(0.75) * t * t;
 }
 
@@ -264,7 +268,8 @@ struct DensLane {
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R rr = f_sqrt(r2);
         const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
-        return in ? q.w * (sig * m4_shape(rr * inv_h)) : R(0);
+        // the mass multiplies unconditionally, so the record stays one 16-byte load
+        return q.w * (in ? sig * m4_shape(rr * inv_h) : R(0));
     }
     // frame of the visited sort cell (shifts from for_cells)
     __device__ __forceinline__ void frame(R sx, R sy, R sz, R& bx, R& by, R& bz) const {
