@@ -262,12 +262,13 @@ struct DensLane {
     }
     // contribution of one candidate; FP32 tests r^2 (W is continuous at 2h),
     // FP64 tests r < 2h exactly as the reference does
-    __device__ __forceinline__ R term(const V4& q, R bx, R by, R bz) const {
+    // (a slot that is not `live` adds exactly zero)
+    __device__ __forceinline__ R term(const V4& q, R bx, R by, R bz, bool live = true) const {
         R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
         if (PER && per) min_image(*gp, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R rr = f_sqrt(r2);
-        const bool in = (sizeof(R) == 4) ? (r2 < th2) : (rr < two_h);
+        const bool in = live && ((sizeof(R) == 4) ? (r2 < th2) : (rr < two_h));
         // the mass multiplies unconditionally, so the record stays one 16-byte load
         return q.w * (in ? sig * m4_shape(rr * inv_h) : R(0));
     }
@@ -358,15 +359,17 @@ __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p
         R bx, by, bz;
         L.frame(sx, sy, sz, bx, by, bz);
         if (!L.emu) {
-            uint32_t k = rg.x;
-            for (; k + 3 < rg.y; k += 4) {       // four loads in flight
-                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1], q2 = w.dens_rec[k + 2], q3 = w.dens_rec[k + 3];
+            // four loads in flight, also for a cell's last partial group: its
+            // spare slots re-read the cell's last record and add zero
+            const uint32_t last = rg.y - 1;
+            for (uint32_t k = rg.x; k < rg.y; k += 4) {
+                const uint32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
+                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k1], q2 = w.dens_rec[k2], q3 = w.dens_rec[k3];
                 s += L.term(q0, bx, by, bz);
-                s += L.term(q1, bx, by, bz);
-                s += L.term(q2, bx, by, bz);
-                s += L.term(q3, bx, by, bz);
+                s += L.term(q1, bx, by, bz, k + 1 < rg.y);
+                s += L.term(q2, bx, by, bz, k + 2 < rg.y);
+                s += L.term(q3, bx, by, bz, k + 3 < rg.y);
             }
-            for (; k < rg.y; k++) s += L.term(w.dens_rec[k], bx, by, bz);
         } else {
             for (uint32_t k = rg.x; k < rg.y; k++) {
                 if (!oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
@@ -597,6 +600,7 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
 }
 
 // ---------------------------------------------------------------- forces
+constexpr int kForceBatch = 4;   // position records in flight per lane (force cell walk)
 // Fused strain rate + forces (sph.hpp:212-258 and 277-317; the Metal
 // strain_forces kernel, sph_force.metal:308-365, fuses the same two loops).
 // For each pair j != i with 0 < r < 2 hbar, hbar = (h_i + h_j)/2:
@@ -687,13 +691,14 @@ struct ForceLane {
     }
 
     // separation of candidate k seen from (bx, by, bz); true when in range
+    // (a slot that is not `live` is never in range)
     __device__ __forceinline__ bool in_range(const V4& q, R bx, R by, R bz, R& rx, R& ry, R& rz, R& r2,
-                                             R& th) const {
+                                             R& th, bool live = true) const {
         rx = bx - q.x; ry = by - q.y; rz = bz - q.z;
         if (PER && per) min_image(*gp, rx, ry, rz);
         r2 = rx * rx + ry * ry + rz * rz;
         th = hi + q.w;
-        return r2 < th * th && r2 != R(0);
+        return live && r2 < th * th && r2 != R(0);
     }
 
     // strain rate, spin, Jaumann rate and deviatoric power (sph.hpp:230-256),
@@ -783,19 +788,22 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                         bx = F.pi.x; by = F.pi.y; bz = F.pi.z;
                     }
                     // candidate k: the other records are read only for pairs in range
-                    auto cand = [&](uint32_t k, const typename Vec4<R>::T& q) {
+                    auto cand = [&](uint32_t k, const typename Vec4<R>::T& q, bool live) {
                         R rx, ry, rz, r2, th;
-                        if (!F.in_range(q, bx, by, bz, rx, ry, rz, r2, th)) return;
+                        if (!F.in_range(q, bx, by, bz, rx, ry, rz, r2, th, live)) return;
                         F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
                                w.frec_sb[k]);
                     };
-                    uint32_t k = rg.x;
-                    for (; k + 1 < rg.y; k += 2) {   // two position records in flight
-                        const typename Vec4<R>::T q0 = w.frec_pos[k], q1 = w.frec_pos[k + 1];
-                        cand(k, q0);
-                        cand(k + 1, q1);
+                    // kForceBatch position records in flight, also for a cell's last partial
+                    // group (its spare slots re-read the cell's last record, not in range)
+                    const uint32_t last = rg.y - 1;
+                    for (uint32_t k = rg.x; k < rg.y; k += kForceBatch) {
+                        typename Vec4<R>::T q[kForceBatch];
+#pragma unroll
+                        for (int b = 0; b < kForceBatch; b++) q[b] = w.frec_pos[min(k + b, last)];
+#pragma unroll
+                        for (int b = 0; b < kForceBatch; b++) cand(min(k + b, last), q[b], k + b < rg.y);
                     }
-                    if (k < rg.y) cand(k, w.frec_pos[k]);
                 });
             }
             dt_i = F.store(c, w, i, info_mat(inf), p);
