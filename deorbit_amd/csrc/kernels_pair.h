@@ -643,14 +643,22 @@ struct ForceLane {
     // neighbour k at separation (rx, ry, rz), r2 = |r|^2 < th^2, th = 2 hbar
     __device__ __forceinline__ void body(const Carried<R>& c, const Work<R>& w, uint32_t k, R rx, R ry, R rz, R r2,
                                          R th, const V4& vj, const V4& tj, const V4& saj, const V4& sbj) {
-        const R r = f_sqrt(r2);
-        if (sizeof(R) == 8 && !(r < th)) return;
+        // FP32: r and 1/r from one reciprocal square root; FP64: IEEE sqrt and divide
+        R r, ir;
+        if constexpr (sizeof(R) == 4) {
+            ir = __builtin_amdgcn_rsqf(r2);
+            r = r2 * ir;
+        } else {
+            r = f_sqrt(r2);
+            if (!(r < th)) return;
+            ir = f_rcp(r);
+        }
         const R hbar = R(0.5) * th;
         const R ihb = f_rcp(hbar);
         const R qq = r * ihb;
         const R sgm = ihb * ihb * ihb * R(1.0 / kPi);
         const R shp = m4_slope(qq);
-        const R fr = sgm * ihb * shp * f_rcp(r);    // dW/dr / r
+        const R fr = sgm * ihb * shp * ir;          // dW/dr / r
         const R gxw = rx * fr, gyw = ry * fr, gzw = rz * fr;
         const R dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
         const R Vj = sbj.z;
@@ -662,10 +670,19 @@ struct ForceLane {
         // one basic block and each record is fetched by a single 16-byte load.
         const R dvr = -(dvx * rx + dvy * ry + dvz * rz);   // v_ij . r_ij
         // Monaghan viscosity on approaching pairs (mu = 0 otherwise gives Pi = 0)
-        const R mu = hbar * smin(dvr, R(0)) * f_rcp(r2 + eps * hbar * hbar);
         const R cbar = R(0.5) * (csig_i + tj.z);
         const R rbar = R(0.5) * (rho_i + tj.x);
-        const R pij = (-alpha * cbar * mu + beta * mu * mu) * f_rcp(rbar);
+        const R den = r2 + eps * hbar * hbar;
+        R mu, pij;
+        if constexpr (sizeof(R) == 4) {
+            // both quotients from one reciprocal: 1/den = rbar/(den rbar), 1/rbar = den/(den rbar)
+            const R inv = f_rcp(den * rbar);
+            mu = hbar * fminf(dvr, 0.0f) * (rbar * inv);
+            pij = (-alpha * cbar * mu + beta * mu * mu) * (den * inv);
+        } else {
+            mu = hbar * smin(dvr, R(0)) * f_rcp(den);
+            pij = (-alpha * cbar * mu + beta * mu * mu) * f_rcp(rbar);
+        }
         const R term = pio + tj.y + pij;
         const R vg = -(dvx * gxw + dvy * gyw + dvz * gzw);  // v_ij . grad W
         const R mj = vj.w;
