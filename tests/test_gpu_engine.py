@@ -260,3 +260,58 @@ def test_profile_reports_every_phase(gpu):
     p = s.profile()
     for ph in ("kick_drift", "sort", "permute", "density", "force", "finish"):
         assert p[ph][1] >= 3 and p[ph][0] > 0, ph
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_batch_with_an_empty_segment(precision, gpu):
+    """A segment that receives no particles (an instance dropped from a sweep)
+    leaves the others exactly as separate runs and reports n = 0."""
+    members = [ics.tiny_impact(seed=1), ics.tiny_impact(R=3, nx=12, ny=8, nz=5, theta=60, seed=2)]
+    sep = []
+    for ic in members:
+        s = dsph.make_system(ic, precision)
+        s.init()
+        s.run(4, 0.0, False)
+        sep.append(s.state(list(FIELDS)))
+        s.close()
+    params = [members[0]["params"], members[0]["params"], members[1]["params"]]
+    b = dsph.System(params, members[0]["materials"], precision)
+    b.add_ic(members[0], seg=np.zeros(len(members[0]["mass"]), np.int32))
+    b.add_ic(members[1], seg=np.full(len(members[1]["mass"]), 2, np.int32))
+    b.seed_damage(members[0]["dx"], 12345, seg=0)
+    b.seed_damage(members[1]["dx"], 12345, seg=2)
+    b.init()
+    b.run(4, 0.0, True)
+    st = b.state(list(FIELDS))
+    n0 = len(members[0]["mass"])
+    for f in FIELDS:
+        assert np.array_equal(st[f][:n0], sep[0][f]), f
+        assert np.array_equal(st[f][n0:], sep[1][f]), f
+    assert b.diagnostics(1)["n"] == 0 and b.diagnostics(2)["n"] == len(members[1]["mass"])
+    dts = b.timestep()
+    assert np.isfinite(dts[0]) and np.isfinite(dts[2])
+    b.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(dsph.FP64, 1e-14), (dsph.MIXED, 1e-6)])
+def test_single_particle_density_has_the_closed_form(precision, tol, gpu):
+    """One particle sees only itself: rho = m W(0, h) = m / (pi h^3).  With
+    eta pi^(1/3) ~ 1.90 > 1 every iteration raises h until the clamp at 2 h_init,
+    where the rounds stop, so rho = m / (pi (2 h_init)^3) and h = 2 h_init
+    (sph.hpp:189-201)."""
+    ic = ics.tiny_impact()
+    p = dict(ic["params"])
+    h0 = p["h_init"]
+    m = 7.8e6
+    s = dsph.System(p, ic["materials"], precision)
+    s.add(np.array([[1.0, 2.0, 3.0]]), np.zeros((1, 3)), np.array([m]), np.array([0.0]),
+          np.array([1], np.int32))
+    s.compute_density_h()
+    assert abs(s.get("hh")[0] - 2 * h0) <= 1e-15 * h0
+    rho = m / (np.pi * (2 * h0) ** 3)
+    assert abs(s.get("rho")[0] - rho) <= tol * rho
+    s.init()
+    s.step(0.0)
+    s.synchronize()
+    assert np.all(s.get("acc") == 0.0) and s.diagnostics()["nonfinite"] == 0
+    s.close()
