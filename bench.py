@@ -217,7 +217,12 @@ def main():
             "traffic": None if rec is None else rec["bytes"],
             "bytes_per_particle": kb["force"], "algorithmic_bytes": kb["force"] * n,
             "mean_launch_ms": round(force_ms, 4), "source_digest": roofline.source_digest(),
-            "step_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5)}
+            "step_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5),
+            # the other roofline of the step: algorithmic FP32 flops against the vector peak
+            "valu": {"flops_per_particle_step": roofline.step_flops(),
+                     "achieved_tflops": round(roofline.step_flops() * value / 1e12, 3),
+                     "peak_tflops": roofline.MI355X_FP32_TFLOPS,
+                     "frac": round(roofline.step_flops() * value / 1e12 / roofline.MI355X_FP32_TFLOPS, 5)}}
     diag = s.diagnostics()
     s.close()
 

@@ -53,6 +53,21 @@ def step_bytes(precision: str = "mixed", passes: int = 3, iterations: int = 2) -
             + (iterations - 1) * k["density"] + k["density_last"] + k["force"] + k["finish"])
 
 
+# Algorithmic floating-point work of one particle-step: the ordered neighbour
+# pairs inside 2h of a uniform lattice particle, (4/3) pi (2 eta)^3 - 1 ~ 73 at
+# eta = 1.3, times the flops of one pair in each pass: ~25 per density term (two
+# iterations per round) and ~172 for the fused strain + momentum + energy pair
+# (kernel slope, velocity gradient, viscosity, artificial stress, deviatoric
+# stress).  Rejected candidates are overhead, not counted.
+PAIRS_PER_PARTICLE = 73
+FLOPS_DENSITY_PAIR = 25
+FLOPS_FORCE_PAIR = 172
+
+
+def step_flops(iterations: int = 2) -> int:
+    return PAIRS_PER_PARTICLE * (iterations * FLOPS_DENSITY_PAIR + FLOPS_FORCE_PAIR)
+
+
 def source_digest() -> str:
     """Short digest of the device sources; ties a recorded counter profile to a build."""
     import hashlib
