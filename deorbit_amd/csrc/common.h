@@ -29,12 +29,26 @@ struct SegPar {
     int32_t per[3];
 };
 
+// Sort cells are columns of width w across x and y, cut along z into slices
+// of width w * kSliceRatio; keys run z fastest, so the slices of one column are
+// consecutive and any z range of a column is one contiguous index range.
+// kSliceBlock consecutive slices of a column (a block, starting at a slice
+// index that is a multiple of kSliceBlock) share one FP32 position frame and
+// one upper bound of h.  A block is 16 h_init tall at the default widths, so
+// the z range a particle reaches in a column (at most ~8.5 h_init) meets at
+// most two blocks.
+constexpr int kSliceBlock = 32;
+constexpr double kSliceRatio = 1.0 / 3.0;
+__device__ __host__ __forceinline__ int block_start(int cz) { return cz - cz % kSliceBlock; }
+// z of a block's frame origin in slices: its middle, so FP32 offsets from it
+// stay within half a block (frame shifts are differences and do not change)
+__device__ __host__ __forceinline__ int block_origin(int cz) { return block_start(cz) + kSliceBlock / 2; }
+
 // Per-segment sort grid and oracle-emulation grid (written on device).
 struct SegGrid {
     double lo[3];
     double cw[3], inv_cw[3];     // cell width per axis (periodic axes: L / dim)
     int32_t dim[3];
-    int32_t bdim[3];
     uint32_t cell_base, ncells;
     // periodic axes: the grid spans the domain [dlo, dhi) of length L with
     // dim whole cells, so the image of a cell is exactly L away
@@ -97,8 +111,9 @@ struct Work {
     uint32_t* cellc;                 // packed (cx | cy << 10 | cz << 20) of the sort cell
     uint32_t* dev_of;                // IC index -> slot in the order before this step's sort
     uint32_t* perm;                  // new slot -> slot in the order before this step's sort
-    uint2* table;                    // per sort cell: [start, end) in the current order
-    float* cell_hmax;                // per sort cell: upper bound of h over its particles
+    uint32_t* cstart;                // per sort cell c: first slot of c in the current order, with one
+                                     // entry past the last cell, so cell c is [cstart[c], cstart[c+1])
+    float* cell_hmax;                // at the key of a block's first slice: upper bound of h in the block
     // Neighbour lists, built by the first density iteration of a step and read
     // by the second iteration and the force pass.  Entries come in quads of
     // four indices; quad q of particle i sits at
@@ -324,15 +339,13 @@ __device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
     return x * q + (x < r ? x : r) + k;
 }
 
-// Morton interleave of three 2-bit coordinates (within a 4x4x4 brick).
-__device__ __host__ __forceinline__ uint32_t brick_morton(uint32_t x, uint32_t y, uint32_t z) {
-    return (x & 1u) | ((y & 1u) << 1) | ((z & 1u) << 2) | ((x & 2u) << 2) | ((y & 2u) << 3) |
-           ((z & 2u) << 4);
-}
+// Column-major key: columns x-major then y, slices z fastest within a column.
 __device__ __host__ __forceinline__ uint32_t cell_key(const SegGrid& g, int cx, int cy, int cz) {
-    uint32_t bl = ((uint32_t)(cx >> 2) * (uint32_t)g.bdim[1] + (uint32_t)(cy >> 2)) * (uint32_t)g.bdim[2] +
-                  (uint32_t)(cz >> 2);
-    return g.cell_base + bl * 64u + brick_morton((uint32_t)cx & 3u, (uint32_t)cy & 3u, (uint32_t)cz & 3u);
+    return g.cell_base + ((uint32_t)cx * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[2] + (uint32_t)cz;
+}
+// Key where the bound of h of the block holding slice cz lives.
+__device__ __host__ __forceinline__ uint32_t block_key(const SegGrid& g, int cx, int cy, int cz) {
+    return cell_key(g, cx, cy, block_start(cz));
 }
 __device__ __host__ __forceinline__ uint32_t cellc_pack(int cx, int cy, int cz) {
     return (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);

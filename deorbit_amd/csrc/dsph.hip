@@ -226,16 +226,14 @@ class Engine : public EngineBase {
     }
 
     // Neighbour lists: nbr_cap entries per particle, in 64-particle blocks.
-    // The automatic capacity is about 1.5x the mean list length of a uniform
-    // lattice at the default skin.
-    // Automatic choice (neighbor_capacity 0): lists for the FP64 path, whose
-    // pair math is costly enough that visiting only listed pairs pays for the
-    // build; cell walks for the FP32 path, where the gathers of a list cost
-    // more than the walk's rejected candidates.
+    // Automatic choice (neighbor_capacity 0): column walks in both paths, which
+    // cost less than building the lists and gathering through them; the
+    // autotuner also tries lists of 128 entries (about 1.5x the mean list
+    // length of a uniform lattice at the default skin).
     int list_capacity() const {
         if (tune_.neighbor_capacity < 0) return 0;
         if (tune_.neighbor_capacity > 0) return tune_.neighbor_capacity;
-        return sizeof(R) == 8 ? 128 : 0;
+        return 0;
     }
     void alloc_lists() {
         const int cap = (list_capacity() + 3) / 4 * 4;
@@ -256,7 +254,7 @@ class Engine : public EngineBase {
         int64_t cap = std::max<int64_t>(1 << 16, (int64_t)tune_.cell_capacity_per_particle * n_ + 65 * (int64_t)nseg_);
         cap = std::min<int64_t>(cap, (int64_t)0xfffffff0);
         capacity_ = (uint32_t)cap;
-        w_.table = dalloc<uint2>(capacity_);
+        w_.cstart = dalloc<uint32_t>((size_t)capacity_ + 1);
         w_.cell_hmax = dalloc<float>(capacity_);
         int bits = 1;
         while (bits < 32 && ((1ull << bits) <= (unsigned long long)capacity_)) bits++;
@@ -414,9 +412,13 @@ class Engine : public EngineBase {
                    nb_sort_, (const uint32_t*)digit_totals_);
             ka = 1 - ka;
         }
+        // first slot of every cell (empty cells included), from the sorted keys
+        const unsigned ncs = std::min<unsigned>(blocks_for((int64_t)capacity_ + 1, 256), 2048u);
+        launch(PH_SORT, "cell_start", k_cell_start, dim3(ncs), dim3(256), (const uint32_t*)keys_[ka], n_, w_.cstart,
+               (const uint32_t*)d_total_cells_);
         const auto th = hmax_target(0);
         launch(PH_PERMUTE, "permute", k_permute<R>, dim3(blocks_for(n_, bp)), dim3(bp), bufs_[cur_],
-               bufs_[1 - cur_], w_, (const uint32_t*)keys_[ka], (const uint32_t*)vals_[ka], th);
+               bufs_[1 - cur_], w_, (const uint32_t*)keys_[ka], (const uint32_t*)vals_[ka], th, w_.nbr_cap > 0 ? 1 : 0);
         fold_parts(PH_PERMUTE, th, blocks_for(n_, bp));
         cur_ = 1 - cur_;
         built_ = true;
@@ -852,7 +854,7 @@ class Engine : public EngineBase {
                 tune_ = nt;
                 // the table is reallocated; everything else is kept
                 release_graphs();
-                for (void* old : {(void*)w_.table, (void*)w_.cell_hmax}) free_ptr(old);
+                for (void* old : {(void*)w_.cstart, (void*)w_.cell_hmax}) free_ptr(old);
                 alloc_table();
                 built_ = false;
                 records_ = false;

@@ -90,9 +90,9 @@ __global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int
     block_reduce<6>(tg, sizeof(SegState) / 8, w.parts, seg, v, fin);
 }
 
-// Sort grid per segment: cubic cells of width cell_factor * h_init over the
-// segment bbox, padded to whole 4x4x4 bricks, laid end to end in one key
-// space.  If the total exceeds the table capacity every width grows by 25 %
+// Sort grid per segment: columns of width w = cell_factor * h_init across x
+// and y cut into slices of w * kSliceRatio along z, over the segment bbox (at
+// most 1001 cells per axis), segments laid end to end in one key space.  If the total exceeds the table capacity every width grows by 25 %
 // until it fits (coarser cells stay correct: stencils are sized in cells).
 // One workgroup of kGridThreads threads: segments are handled a chunk at a
 // time, one per thread, and a block scan lays their key ranges end to end.
@@ -112,13 +112,12 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
             unsigned long long nc = 0;
             if (s < nseg) {
                 SegGrid& g = grid[s];
-                double lo[3], ext[3], emax = 0.0;
+                double lo[3], ext[3];
                 for (int a = 0; a < 3; a++) {
                     double l = ord_dec(st[s].bmin[a]), h = ord_dec(st[s].bmax[a]);
                     if (!(l <= h)) { l = 0.0; h = 0.0; }          // segment without finite particles
                     lo[a] = l;
                     ext[a] = h - l;
-                    emax = ext[a] > emax ? ext[a] : emax;
                 }
                 // periodic axes span the domain instead of the particle bbox
                 // (positions are not wrapped; cell_coord clamps strays into the
@@ -130,15 +129,15 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
                     if (per[a]) {
                         lo[a] = par[s].dlo[a];
                         ext[a] = L;
-                        emax = ext[a] > emax ? ext[a] : emax;
                     }
                 }
-                double cw = cell_factor * par[s].h_init * scale;
-                if (emax / cw > 1000.0) cw = emax / 1000.0;         // at most 1001 cells per axis
-                // 64 cells per brick plus one trailing "park" cell that collects
-                // particles with a non-finite position (never visited by a stencil)
-                nc = 64;
+                const double w = cell_factor * par[s].h_init * scale;
+                // one trailing "park" cell collects particles with a non-finite
+                // position (never visited by a stencil)
+                nc = 1;
                 for (int a = 0; a < 3; a++) {
+                    double cw = (a == 2) ? w * kSliceRatio : w;
+                    if (ext[a] / cw > 1000.0) cw = ext[a] / 1000.0;   // at most 1001 cells per axis
                     g.lo[a] = lo[a];
                     g.per[a] = per[a];
                     if (per[a]) {
@@ -154,10 +153,8 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
                         g.cw[a] = cw;
                     }
                     g.inv_cw[a] = 1.0 / g.cw[a];
-                    g.bdim[a] = (g.dim[a] + 3) / 4;
-                    nc *= (unsigned long long)g.bdim[a];
                 }
-                nc += 1;
+                nc += (unsigned long long)g.dim[0] * (unsigned long long)g.dim[1] * (unsigned long long)g.dim[2];
                 g.ncells = (uint32_t)nc;
             }
             // inclusive scan of the chunk's cell counts
@@ -192,7 +189,7 @@ __device__ __forceinline__ int cell_coord(const SegGrid& g, int a, double v) {
     return c;
 }
 
-// Cell key of every particle, and the clearing of the cell table (grid-stride).
+// Cell key of every particle, and the clearing of the bounds of h (grid-stride).
 template <class R>
 __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t* keys, uint32_t* vals,
                                               const uint32_t* total_cells) {
@@ -213,9 +210,24 @@ __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t*
     }
     const uint32_t total = *total_cells;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < total; k += stride) {
-        w.table[k] = make_uint2(0u, 0u);
+    for (uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < total; k += stride)
         w.cell_hmax[k] = 0.0f;
+}
+
+// First slot of every cell in the sorted keys (a lower bound per cell, so empty
+// cells get the slot where they would start), for cells 0 .. total inclusive.
+__global__ __launch_bounds__(256) void k_cell_start(const uint32_t* skeys, int64_t n, uint32_t* cstart,
+                                                   const uint32_t* total_cells) {
+    const uint32_t total = *total_cells;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); c <= total; c += stride) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (skeys[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+        cstart[c] = (uint32_t)lo;
     }
 }
 
@@ -336,14 +348,15 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, 
 
 // ---------------------------------------------------------------- permutation
 // Gather the carried state into sorted order, carry the last force pass's
-// dudt/dSdt along as the "old" derivatives (sph.hpp:358-359), record each
-// cell's [start, end) range, and write the density-pass neighbour records.
-// Positions in the mixed path are stored as FP32 offsets from the particle's
-// own sort-cell corner (pair separations are then formed from small numbers);
-// the FP64 path keeps absolute positions.
+// dudt/dSdt along as the "old" derivatives (sph.hpp:358-359), and write the
+// density-pass neighbour records.  Positions in the mixed path are stored as
+// FP32 offsets from the corner of the particle's own column (x, y) and the
+// middle of its slice block (z), so pair separations are formed from numbers
+// below a few h; the FP64 path keeps absolute positions.
 template <class R>
 __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst, Work<R> w,
-                                                 const uint32_t* keys, const uint32_t* vals, SegTargets<1> tg) {
+                                                 const uint32_t* keys, const uint32_t* vals, SegTargets<1> tg,
+                                                 int list_bounds) {
     using V4 = typename Vec4<R>::T;
     const int64_t n = w.n;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,23 +385,22 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
         for (int k = 0; k < 6; k++) w.dSdt_old[(size_t)k * n + i] = w.dSdt[(size_t)k * n + s];
 
         const uint32_t key = keys[i];
-        const uint32_t prev = (i > 0) ? keys[i - 1] : 0xffffffffu;
-        const uint32_t next = (i + 1 < n) ? keys[i + 1] : 0xffffffffu;
-        if (key != prev) w.table[key].x = (uint32_t)i;
-        if (key != next) w.table[key].y = (uint32_t)(i + 1);
         const SegGrid& g = w.grid[seg];
         w.h0[i] = h;
         if (key != g.cell_base + g.ncells - 1u) {
-            // per-cell bound of h at the start of the step (list build radius;
-            // the density epilogue raises it to the final h)
-            atomicMax((unsigned int*)&w.cell_hmax[key], __float_as_uint(float_at_least((double)h)));
             const int cx = cell_coord(g, 0, x), cy = cell_coord(g, 1, y), cz = cell_coord(g, 2, z);
+            // per-block bound of h at the start of the step, read only by the
+            // neighbour-list build (the density epilogue raises it to the final
+            // h that the force pass culls with)
+            if (list_bounds)
+                atomicMax((unsigned int*)&w.cell_hmax[block_key(g, cx, cy, cz)],
+                          __float_as_uint(float_at_least((double)h)));
             w.cellc[i] = cellc_pack(cx, cy, cz);
             V4 rec;
             if (sizeof(R) == 4) {
                 rec.x = (R)(x - (g.lo[0] + (double)cx * g.cw[0]));
                 rec.y = (R)(y - (g.lo[1] + (double)cy * g.cw[1]));
-                rec.z = (R)(z - (g.lo[2] + (double)cz * g.cw[2]));
+                rec.z = (R)(z - (g.lo[2] + (double)block_origin(cz) * g.cw[2]));
             } else {
                 rec.x = (R)x; rec.y = (R)y; rec.z = (R)z;
             }

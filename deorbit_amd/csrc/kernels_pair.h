@@ -13,7 +13,9 @@
 
 namespace dsph {
 
-// Own-cell offset of particle i (FP64 path: derived from absolute position).
+// Offset of particle i from its own frame: the corner of its column (x, y)
+// and the middle of its slice block (z).  FP32 path: the density record; FP64 path:
+// derived from the absolute position.
 template <class R>
 __device__ __forceinline__ void own_offset(const Carried<R>& c, const SegGrid& g, int64_t i, uint32_t cc,
                                            const typename Vec4<R>::T& rec, R& ox, R& oy, R& oz) {
@@ -23,8 +25,14 @@ __device__ __forceinline__ void own_offset(const Carried<R>& c, const SegGrid& g
         const int cx = cc & 1023u, cy = (cc >> 10) & 1023u, cz = cc >> 20;
         ox = (R)(c.x[i] - (g.lo[0] + (double)cx * g.cw[0]));
         oy = (R)(c.y[i] - (g.lo[1] + (double)cy * g.cw[1]));
-        oz = (R)(c.z[i] - (g.lo[2] + (double)cz * g.cw[2]));
+        oz = (R)(c.z[i] - (g.lo[2] + (double)block_origin(cz) * g.cw[2]));
     }
+}
+// Offset from the corner of the own slice, given the offset oz from the own
+// block's frame origin (what the distance culling of z slices measures from).
+template <class R>
+__device__ __forceinline__ R slice_offset(const SegGrid& g, int cz, R oz) {
+    return oz - (R)(cz - block_origin(cz)) * (R)g.cw[2];
 }
 
 // Distance along one axis from offset o (in [0, cw)) to the stencil cell d.
@@ -48,17 +56,20 @@ __device__ __forceinline__ void min_image(const SegGrid& g, R& rx, R& ry, R& rz)
 
 // The cells a stencil walk visits along one axis.  Offset d names the cell
 // c + d, wrapped on a periodic axis (whose cells tile the period exactly, so a
-// wrapped cell's image sits d cell widths away); the frame shift d * cw is the
-// corner of the visited image relative to the own cell's corner.  A periodic
-// axis with fewer cells than the stencil is walked once over all its cells
-// (offsets -(dim-1)/2 .. dim-1-(dim-1)/2) without culling.
+// wrapped cell's image sits d cell widths away).  The frame shift is the
+// corner of the visited image's frame relative to the own frame's corner;
+// frames are cells along x and y and blocks of kSliceBlock slices along z
+// (blk).  A periodic axis with fewer cells than the stencil is walked once
+// over all its cells (offsets -(dim-1)/2 .. dim-1-(dim-1)/2) without culling.
+// o is the offset from the own cell's corner (for the distance culling).
 template <class R, bool PER>
 struct AxisWalk {
-    int c, dim, lo, hi, per;
+    int c, dim, lo, hi, per, blk;
     bool cull;
     R o, cw;
     __device__ __forceinline__ void init(const SegGrid& g, int a, int cc, R oo, double reach) {
         c = cc; dim = g.dim[a]; per = PER && g.per[a]; o = oo; cw = (R)g.cw[a];
+        blk = (a == 2) ? kSliceBlock : 1;
         int rad = (int)ceil(reach * g.inv_cw[a]);
         rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
         cull = true;
@@ -78,7 +89,9 @@ struct AxisWalk {
         } else if ((unsigned)n >= (unsigned)dim) {
             return false;
         }
-        shift = (R)d * cw;
+        // image frame corner: frame start of n, moved by the wrap (c + d - n)
+        const int fs = n - n % blk, fc = c - c % blk;
+        shift = (R)(fs + (c + d - n) - fc) * cw;
         gap = cull ? cell_gap(d, o, cw) : R(0);
         return true;
     }
@@ -180,13 +193,13 @@ __device__ __forceinline__ void eos_epilogue(const Carried<R>& c, const Work<R>&
     const R sc = dsc * ir2;
     sa.x = s6[0] * sc; sa.y = s6[1] * sc; sa.z = s6[2] * sc; sa.w = s6[3] * sc;
     sb.x = s6[4] * sc; sb.y = s6[5] * sc; sb.z = mi / rho; sb.w = R(0);
-    // per-cell upper bound of h for the force stencil (positive floats order as integers)
+    // per-block upper bound of h for the force stencil (positive floats order as integers)
     const uint32_t cc = w.cellc[i];
     const SegGrid& g = w.grid[info_seg(c.info[i])];
     if (isfinite((double)ox)) {
         float hf = (float)h;
         if ((double)hf < (double)h) hf = __uint_as_float(__float_as_uint(hf) + 1u);   // round up
-        atomicMax((unsigned int*)&w.cell_hmax[cell_key(g, cc & 1023u, (cc >> 10) & 1023u, cc >> 20)],
+        atomicMax((unsigned int*)&w.cell_hmax[block_key(g, cc & 1023u, (cc >> 10) & 1023u, cc >> 20)],
                   __float_as_uint(hf));
     }
     w.frec_pos[i] = a;
@@ -205,25 +218,29 @@ __device__ __forceinline__ uint32_t quad_get(const uint4& v, int s) {
     return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
 }
 
-// Position of particle i relative to the frame of neighbour k's sort cell:
-// FP32 path, own-cell offset shifted by the cell delta (so the pair separation
-// is formed from numbers below one cell width), taking on a periodic axis the
-// image the stencil walk uses; FP64 path, absolute position.
+// Position of particle i relative to the frame of neighbour k: FP32 path, the
+// own frame offset shifted by the frame delta (cells along x and y, slice
+// blocks along z), taking on a periodic axis the image the stencil walk uses;
+// FP64 path, absolute position.
 template <class R, bool PER>
 __device__ __forceinline__ void rel_base(const Work<R>& w, const SegGrid& g, uint32_t k, int cx, int cy, int cz,
                                          R ox, R oy, R oz, R ax, R ay, R az, R& bx, R& by, R& bz) {
     if (sizeof(R) == 4) {
         const uint32_t ck = w.cellc[k];
-        int d[3] = {(int)(ck & 1023u) - cx, (int)((ck >> 10) & 1023u) - cy, (int)(ck >> 20) - cz};
+        const int cj[3] = {(int)(ck & 1023u), (int)((ck >> 10) & 1023u), (int)(ck >> 20)};
+        const int ci[3] = {cx, cy, cz};
         R sh[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
+            const int d = cj[a] - ci[a];
+            int wrap = 0;                  // cells moved to reach the image
             if (PER && g.per[a]) {
                 const int m = g.dim[a], lo = -((m - 1) / 2), hi = m - 1 - (m - 1) / 2;
-                if (d[a] < lo) d[a] += m;
-                else if (d[a] > hi) d[a] -= m;
+                if (d < lo) wrap = m;
+                else if (d > hi) wrap = -m;
             }
-            sh[a] = (R)d[a] * (R)g.cw[a];
+            const int fj = (a == 2) ? block_start(cj[a]) : cj[a], fi = (a == 2) ? block_start(ci[a]) : ci[a];
+            sh[a] = (R)(fj - fi + wrap) * (R)g.cw[a];
         }
         bx = ox - sh[0]; by = oy - sh[1]; bz = oz - sh[2];
     } else {
@@ -238,6 +255,7 @@ struct DensLane {
     using V4 = typename Vec4<R>::T;
     V4 me;                 // own density record
     R h, ox, oy, oz, two_h, th2, inv_h, sig;
+    R ozs;                 // offset from the own slice's corner (distance culling)
     int cx, cy, cz;
     bool emu;              // 2h may exceed the reference's cell width this round
     bool per;              // some axis of the segment is periodic
@@ -246,6 +264,7 @@ struct DensLane {
 
     __device__ void setup(const Carried<R>& c, const SegPar& p, const SegGrid& g, int64_t i, uint32_t cc) {
         cx = cc & 1023u; cy = (cc >> 10) & 1023u; cz = cc >> 20;
+        ozs = slice_offset<R>(g, cz, oz);
         gp = &g;
         per = PER && (g.per[0] || g.per[1] || g.per[2]);
         two_h = R(2) * h;
@@ -272,7 +291,7 @@ struct DensLane {
         // the mass multiplies unconditionally, so the record stays one 16-byte load
         return q.w * (in ? sig * m4_shape(rr * inv_h) : R(0));
     }
-    // frame of the visited sort cell (shifts from for_cells)
+    // frame of a visited piece (shifts from for_pieces)
     __device__ __forceinline__ void frame(R sx, R sy, R sz, R& bx, R& by, R& bz) const {
         if (sizeof(R) == 4) {
             bx = ox - sx; by = oy - sy; bz = oz - sz;
@@ -282,98 +301,144 @@ struct DensLane {
     }
 };
 
-// Visit the sort cells whose nearest point is closer than `reach` to particle
-// i, in lexicographic cell order; fn(key, sx, sy, sz, gap^2) per cell, with
-// (sx, sy, sz) the frame shift of the visited cell (see AxisWalk).
-template <class R, bool PER, class F>
-__device__ __forceinline__ void for_cells(const SegGrid& g, int cx, int cy, int cz, R ox, R oy, R oz, R reach,
-                                          F&& fn) {
-    const R reach2 = reach * reach;
+// Floor to int in the working precision.
+__device__ __forceinline__ int ifloor(float x) { return (int)floorf(x); }
+__device__ __forceinline__ int ifloor(double x) { return (int)floor(x); }
+// Slices added at both ends of a column's z range so that rounding never
+// drops a slice within reach (the extra candidates fail the range test).
+constexpr float kSliceMargin = 1e-3f;
+
+// Visit the candidates of particle i as pieces: runs [k0, k1) of the sorted
+// order with one frame shift (sx, sy) across x and y and, along z, sz0 for
+// slots below ksplit and sz1 from ksplit on (see AxisWalk);
+// fn(k0, k1, ksplit, sx, sy, sz0, sz1, reach) per piece.  reach_max bounds
+// the reach; with HB the reach of a piece is reach_of(its bound of h) and the
+// piece is culled with it (else reach is reach_max).  (ox, oy) are offsets
+// from the own column's corner, ozs from the own slice's corner.
+//
+// Without periodic axes the walk goes column by column: every column whose
+// nearest point is within reach, and in it one contiguous run of the slices
+// within reach along z (at most two slice blocks, hence one split).  A wave's
+// lanes sit in one column and walk the same column sequence with runs of
+// similar length, and the z range is cut to the lane's own reach rather than
+// to whole cubic cells.  Periodic segments walk cell by cell.
+template <class R, bool PER, bool HB, class RF, class F>
+__device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, int cx, int cy, int cz, R ox, R oy,
+                                           R ozs, R reach_max, RF&& reach_of, F&& fn) {
+    const R rmax2 = reach_max * reach_max;
     if constexpr (!PER) {
-        // no periodic axis: every axis has the same cell width, one radius
-        // serves all three and the walk state fits fewer registers
-        const R cw = (R)g.cw[0];
-        int rad = (int)ceil((double)reach * g.inv_cw[0]);
-        rad = rad < 1 ? 1 : (rad > kMaxStencil ? kMaxStencil : rad);
-        for (int dx = -rad; dx <= rad; dx++) {
+        const R wx = (R)g.cw[0], wy = (R)g.cw[1], wz = (R)g.cw[2], iwz = (R)g.inv_cw[2];
+        int rx = (int)ceil((double)reach_max * g.inv_cw[0]), ry = (int)ceil((double)reach_max * g.inv_cw[1]);
+        rx = rx < 1 ? 1 : (rx > kMaxStencil ? kMaxStencil : rx);
+        ry = ry < 1 ? 1 : (ry > kMaxStencil ? kMaxStencil : ry);
+        const int fi = block_start(cz), zmax = g.dim[2] - 1;
+        for (int dx = -rx; dx <= rx; dx++) {
             const int nx = cx + dx;
             if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
-            const R gx = cell_gap(dx, ox, cw);
+            const R gx = cell_gap(dx, ox, wx);
             const R gx2 = gx * gx;
-            if (gx2 >= reach2) continue;
-            for (int dy = -rad; dy <= rad; dy++) {
+            if (gx2 >= rmax2) continue;
+            for (int dy = -ry; dy <= ry; dy++) {
                 const int ny = cy + dy;
                 if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
-                const R gy = cell_gap(dy, oy, cw);
+                const R gy = cell_gap(dy, oy, wy);
                 const R gxy = gx2 + gy * gy;
-                if (gxy >= reach2) continue;
-                for (int dz = -rad; dz <= rad; dz++) {
-                    const int nz = cz + dz;
-                    if ((unsigned)nz >= (unsigned)g.dim[2]) continue;
-                    const R gz = cell_gap(dz, oz, cw);
-                    const R gg = gxy + gz * gz;
-                    if (gg >= reach2) continue;
-                    fn(cell_key(g, nx, ny, nz), (R)dx * cw, (R)dy * cw, (R)dz * cw, gg);
+                if (gxy >= rmax2) continue;
+                const uint32_t col = cell_key(g, nx, ny, 0);
+                R rz = f_sqrt(rmax2 - gxy);
+                int z0 = cz + ifloor((ozs - rz) * iwz - (R)kSliceMargin);
+                int z1 = cz + ifloor((ozs + rz) * iwz + (R)kSliceMargin);
+                z0 = z0 < 0 ? 0 : z0;
+                z1 = z1 > zmax ? zmax : z1;
+                if (z0 > z1) continue;
+                const int b0 = block_start(z0), b1 = block_start(z1);
+                R rb = reach_max;
+                if constexpr (HB) {
+                    // the larger bound of h of the (at most two) blocks met
+                    const R hb0 = (R)w.cell_hmax[col + b0];
+                    const R hb1 = (b1 != b0) ? (R)w.cell_hmax[col + b1] : hb0;
+                    rb = reach_of(hb0 > hb1 ? hb0 : hb1);
+                    const R rb2 = rb * rb;
+                    if (gxy >= rb2) continue;
+                    rz = f_sqrt(rb2 - gxy);
+                    const int qa = cz + ifloor((ozs - rz) * iwz - (R)kSliceMargin);
+                    const int qe = cz + ifloor((ozs + rz) * iwz + (R)kSliceMargin);
+                    z0 = qa > z0 ? qa : z0;
+                    z1 = qe < z1 ? qe : z1;
+                    if (z0 > z1) continue;
                 }
+                const uint32_t k0 = w.cstart[col + z0], k1 = w.cstart[col + z1 + 1];
+                if (k0 >= k1) continue;
+                const int f0 = block_start(z0), f1 = block_start(z1);
+                const uint32_t ks = (f1 != f0) ? w.cstart[col + f1] : k1;
+                fn(k0, k1, ks, (R)dx * wx, (R)dy * wy, (R)(f0 - fi) * wz, (R)(f1 - fi) * wz, rb);
             }
         }
         return;
     }
     AxisWalk<R, PER> X, Y, Z;
-    X.init(g, 0, cx, ox, (double)reach);
-    Y.init(g, 1, cy, oy, (double)reach);
-    Z.init(g, 2, cz, oz, (double)reach);
+    X.init(g, 0, cx, ox, (double)reach_max);
+    Y.init(g, 1, cy, oy, (double)reach_max);
+    Z.init(g, 2, cz, ozs, (double)reach_max);
     for (int dx = X.lo; dx <= X.hi; dx++) {
         int nx;
         R sx, gx;
         if (!X.cell(dx, nx, sx, gx)) continue;
         const R gx2 = gx * gx;
-        if (gx2 >= reach2) continue;
+        if (gx2 >= rmax2) continue;
         for (int dy = Y.lo; dy <= Y.hi; dy++) {
             int ny;
             R sy, gy;
             if (!Y.cell(dy, ny, sy, gy)) continue;
             const R gxy = gx2 + gy * gy;
-            if (gxy >= reach2) continue;
+            if (gxy >= rmax2) continue;
             for (int dz = Z.lo; dz <= Z.hi; dz++) {
                 int nz;
                 R sz, gz;
                 if (!Z.cell(dz, nz, sz, gz)) continue;
                 const R gg = gxy + gz * gz;
-                if (gg >= reach2) continue;
-                fn(cell_key(g, nx, ny, nz), sx, sy, sz, gg);
+                if (gg >= rmax2) continue;
+                R rb = reach_max;
+                if constexpr (HB) {
+                    rb = reach_of((R)w.cell_hmax[block_key(g, nx, ny, nz)]);
+                    if (gg >= rb * rb) continue;
+                }
+                const uint32_t key = cell_key(g, nx, ny, nz);
+                const uint32_t k0 = w.cstart[key], k1 = w.cstart[key + 1];
+                if (k0 < k1) fn(k0, k1, k1, sx, sy, sz, sz, rb);
             }
         }
     }
 }
 
-// Sum over the sort cells within 2h of particle i, in lexicographic cell order
-// and cell-table order inside a cell.
+// Sum over the candidates within 2h of particle i (piece order, sorted order
+// inside a piece).
 template <class R, bool PER>
 __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p, const SegGrid& g,
                           const DensLane<R, PER>& L) {
     using V4 = typename Vec4<R>::T;
     R s = R(0);
-    for_cells<R, PER>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, L.two_h, [&](uint32_t key, R sx, R sy, R sz, R) {
-        const uint2 rg = w.table[key];
-        R bx, by, bz;
-        L.frame(sx, sy, sz, bx, by, bz);
+    for_pieces<R, PER, false>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h, [](R) { return R(0); },
+                              [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1, R) {
+        R bx, by, bz0, bz1;
+        L.frame(sx, sy, sz0, bx, by, bz0);
+        L.frame(sx, sy, sz1, bx, by, bz1);
         if (!L.emu) {
-            // four loads in flight, also for a cell's last partial group: its
-            // spare slots re-read the cell's last record and add zero
-            const uint32_t last = rg.y - 1;
-            for (uint32_t k = rg.x; k < rg.y; k += 4) {
-                const uint32_t k1 = min(k + 1, last), k2 = min(k + 2, last), k3 = min(k + 3, last);
-                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k1], q2 = w.dens_rec[k2], q3 = w.dens_rec[k3];
-                s += L.term(q0, bx, by, bz);
-                s += L.term(q1, bx, by, bz, k + 1 < rg.y);
-                s += L.term(q2, bx, by, bz, k + 2 < rg.y);
-                s += L.term(q3, bx, by, bz, k + 3 < rg.y);
+            // four loads in flight, also for a piece's last partial group: its
+            // spare slots re-read the piece's last record and add zero
+            const uint32_t last = k1 - 1;
+            for (uint32_t k = k0; k < k1; k += 4) {
+                const uint32_t ka = min(k + 1, last), kb = min(k + 2, last), kc = min(k + 3, last);
+                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[ka], q2 = w.dens_rec[kb], q3 = w.dens_rec[kc];
+                s += L.term(q0, bx, by, k < ks ? bz0 : bz1);
+                s += L.term(q1, bx, by, ka < ks ? bz0 : bz1, k + 1 < k1);
+                s += L.term(q2, bx, by, kb < ks ? bz0 : bz1, k + 2 < k1);
+                s += L.term(q3, bx, by, kc < ks ? bz0 : bz1, k + 3 < k1);
             }
         } else {
-            for (uint32_t k = rg.x; k < rg.y; k++) {
+            for (uint32_t k = k0; k < k1; k++) {
                 if (!oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
-                s += L.term(w.dens_rec[k], bx, by, bz);
+                s += L.term(w.dens_rec[k], bx, by, k < ks ? bz0 : bz1);
             }
         }
     });
@@ -539,15 +604,16 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
             // entries are gathered four at a time and stored as one 16-byte quad;
             // unused lanes of a last partial quad keep valid indices
             uint4 quad = make_uint4((uint32_t)i, (uint32_t)i, (uint32_t)i, (uint32_t)i);
-            for_cells<R, PER>(g, L.cx, L.cy, L.cz, L.ox, L.oy, L.oz, sk * (L.h + smax(L.h, hm)),
-                         [&](uint32_t key, R sx, R sy, R sz, R gg) {
-                const R rb = sk * (L.h + smax(L.h, (R)w.cell_hmax[key]));
+            for_pieces<R, PER, true>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, sk * (L.h + smax(L.h, hm)),
+                                     [&](R hb) { return sk * (L.h + smax(L.h, hb)); },
+                                     [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1,
+                                         R rb) {
                 const R rb2 = rb * rb;
-                if (gg >= rb2) return;
-                const uint2 rg = w.table[key];
-                R bx, by, bz;
-                L.frame(sx, sy, sz, bx, by, bz);
+                R bx, by, bz0, bz1;
+                L.frame(sx, sy, sz0, bx, by, bz0);
+                L.frame(sx, sy, sz1, bx, by, bz1);
                 auto visit = [&](uint32_t k, const V4& q) {
+                    const R bz = k < ks ? bz0 : bz1;
                     R rx = bx - q.x, ry = by - q.y, rz = bz - q.z;
                     if (PER && L.per) min_image(g, rx, ry, rz);
                     const R r2 = rx * rx + ry * ry + rz * rz;
@@ -565,13 +631,13 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
                     if (L.emu && !oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) return;
                     s += L.term(q, bx, by, bz);
                 };
-                uint32_t k = rg.x;
-                for (; k + 1 < rg.y; k += 2) {       // two loads in flight
+                uint32_t k = k0;
+                for (; k + 1 < k1; k += 2) {         // two loads in flight
                     const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[k + 1];
                     visit(k, q0);
                     visit(k + 1, q1);
                 }
-                if (k < rg.y) visit(k, w.dens_rec[k]);
+                if (k < k1) visit(k, w.dens_rec[k]);
             });
             if (cnt > (uint32_t)cap) {
                 cnt = kNbrOverflow;
@@ -617,6 +683,7 @@ struct ForceLane {
     R hi, rho_i, pio, csig_i, rart_i;
     R alpha, beta, eps, eps_as, alpha_u, inv_fdp, ui, Pi;
     R ox, oy, oz;
+    R ozs;                 // offset from the own slice's corner (distance culling)
     int cx, cy, cz;
     bool per;              // some axis of the segment is periodic
     const SegGrid* gp;
@@ -792,34 +859,34 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
             ForceLane<R, PER> F;
             F.setup(c, w, p, g, i, cc);
             own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
+            F.ozs = slice_offset<R>(g, F.cz, F.oz);
             if (isfinite((double)F.ox) && isfinite((double)F.oy) && isfinite((double)F.oz)) {
-                for_cells<R, PER>(g, F.cx, F.cy, F.cz, F.ox, F.oy, F.oz, F.hi + (R)w.st[seg].hmax_final,
-                             [&](uint32_t key, R sx, R sy, R sz, R gg) {
-                    const R reach = F.hi + (R)w.cell_hmax[key];
-                    if (gg >= reach * reach) return;
-                    const uint2 rg = w.table[key];
-                    R bx, by, bz;
+                for_pieces<R, PER, true>(g, w, F.cx, F.cy, F.cz, F.ox, F.oy, F.ozs,
+                                         F.hi + (R)w.st[seg].hmax_final, [&](R hb) { return F.hi + hb; },
+                                         [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1,
+                                             R) {
+                    R bx, by, bz0, bz1;
                     if (sizeof(R) == 4) {
-                        bx = F.ox - sx; by = F.oy - sy; bz = F.oz - sz;
+                        bx = F.ox - sx; by = F.oy - sy; bz0 = F.oz - sz0; bz1 = F.oz - sz1;
                     } else {
-                        bx = F.pi.x; by = F.pi.y; bz = F.pi.z;
+                        bx = F.pi.x; by = F.pi.y; bz0 = bz1 = F.pi.z;
                     }
                     // candidate k: the other records are read only for pairs in range
                     auto cand = [&](uint32_t k, const typename Vec4<R>::T& q, bool live) {
                         R rx, ry, rz, r2, th;
-                        if (!F.in_range(q, bx, by, bz, rx, ry, rz, r2, th, live)) return;
+                        if (!F.in_range(q, bx, by, k < ks ? bz0 : bz1, rx, ry, rz, r2, th, live)) return;
                         F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
                                w.frec_sb[k]);
                     };
-                    // kForceBatch position records in flight, also for a cell's last partial
-                    // group (its spare slots re-read the cell's last record, not in range)
-                    const uint32_t last = rg.y - 1;
-                    for (uint32_t k = rg.x; k < rg.y; k += kForceBatch) {
+                    // kForceBatch position records in flight, also for a piece's last partial
+                    // group (its spare slots re-read the piece's last record, not in range)
+                    const uint32_t last = k1 - 1;
+                    for (uint32_t k = k0; k < k1; k += kForceBatch) {
                         typename Vec4<R>::T q[kForceBatch];
 #pragma unroll
                         for (int b = 0; b < kForceBatch; b++) q[b] = w.frec_pos[min(k + b, last)];
 #pragma unroll
-                        for (int b = 0; b < kForceBatch; b++) cand(min(k + b, last), q[b], k + b < rg.y);
+                        for (int b = 0; b < kForceBatch; b++) cand(min(k + b, last), q[b], k + b < k1);
                     }
                 });
             }
