@@ -34,11 +34,12 @@ struct SegPar {
 // consecutive and any z range of a column is one contiguous index range.
 // kSliceBlock consecutive slices of a column (a block, starting at a slice
 // index that is a multiple of kSliceBlock) share one FP32 position frame and
-// one upper bound of h.  A block is 16 h_init tall at the default widths, so
-// the z range a particle reaches in a column (at most ~8.5 h_init) meets at
-// most two blocks.
-constexpr int kSliceBlock = 32;
-constexpr double kSliceRatio = 1.0 / 3.0;
+// one upper bound of h.  The z range a particle reaches in a column, at most
+// 2 * reach / (slice width) + 3 slices with reach <= (1 + list skin) * 4 h_init
+// (h <= 2 h_init), meets at most two blocks as long as the tuning keeps the
+// column width >= h_init and the skin <= 0.25: 2 * 5 / 0.25 + 3 <= 48 + 1.
+constexpr int kSliceBlock = 48;
+constexpr double kSliceRatio = 1.0 / 4.0;
 __device__ __host__ __forceinline__ int block_start(int cz) { return cz - cz % kSliceBlock; }
 // z of a block's frame origin in slices: its middle, so FP32 offsets from it
 // stay within half a block (frame shifts are differences and do not change)
@@ -69,7 +70,7 @@ struct SegState {
     long long steps;
     int32_t active[4];
     int32_t rounds;
-    int32_t bad;                           // non-zero: capacity or numeric trouble
+    int32_t bad;                           // non-zero: capacity, numeric or reach trouble
     double hmax_final;
     uint32_t list_bad;                     // non-zero: some h outgrew the neighbour-list skin
     uint32_t pad_;

@@ -629,6 +629,9 @@ class Engine : public EngineBase {
             if (st[s].bad & 1) throw Err{DSPH_E_RANGE, "cell table capacity exceeded"};
             if (st[s].bad & 4) throw Err{DSPH_E_NUMERIC, "non-finite particle position in segment " + std::to_string(s)};
             if (st[s].bad & 2) throw Err{DSPH_E_NUMERIC, "non-finite or negative time step in segment " + std::to_string(s)};
+            if (st[s].bad & 8)
+                throw Err{DSPH_E_RANGE, "a smoothing length far above 2 h_init reaches past two slice blocks; "
+                                        "use a larger cell_factor"};
         }
         return 0;
     }
@@ -832,12 +835,13 @@ class Engine : public EngineBase {
         if (t.block_pair) { if (!ok_block(t.block_pair)) throw Err{DSPH_E_ARG, "block_pair must be 64, 128 or 256"}; nt.block_pair = t.block_pair; }
         if (t.block_point) { if (!ok_block(t.block_point)) throw Err{DSPH_E_ARG, "block_point must be 64, 128 or 256"}; nt.block_point = t.block_point; }
         if (t.cell_factor != 0.0) {
-            if (!(t.cell_factor >= 0.25 && t.cell_factor <= 8.0)) throw Err{DSPH_E_ARG, "cell_factor must be in [0.25, 8]"};
+            // (a column at least h_init wide keeps every z range within two slice blocks)
+            if (!(t.cell_factor >= 1.0 && t.cell_factor <= 8.0)) throw Err{DSPH_E_ARG, "cell_factor must be in [1, 8]"};
             nt.cell_factor = t.cell_factor;
         }
         nt.xcd_remap = t.xcd_remap ? 1 : 0;
         if (t.list_skin != 0.0) {
-            if (!(t.list_skin > 0.0 && t.list_skin <= 1.0)) throw Err{DSPH_E_ARG, "list_skin must be in (0, 1]"};
+            if (!(t.list_skin > 0.0 && t.list_skin <= 0.25)) throw Err{DSPH_E_ARG, "list_skin must be in (0, 0.25]"};
             nt.list_skin = t.list_skin;
         }
         if (t.neighbor_capacity) {

@@ -318,7 +318,8 @@ constexpr float kSliceMargin = 1e-3f;
 //
 // Without periodic axes the walk goes column by column: every column whose
 // nearest point is within reach, and in it one contiguous run of the slices
-// within reach along z (at most two slice blocks, hence one split).  A wave's
+// within reach along z (at most two slice blocks within the tuning limits,
+// hence one frame split).  A wave's
 // lanes sit in one column and walk the same column sequence with runs of
 // similar length, and the z range is cut to the lane's own reach rather than
 // to whole cubic cells.  Periodic segments walk cell by cell.
@@ -351,12 +352,19 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                 z0 = z0 < 0 ? 0 : z0;
                 z1 = z1 > zmax ? zmax : z1;
                 if (z0 > z1) continue;
-                const int b0 = block_start(z0), b1 = block_start(z1);
+                const int f0 = block_start(z0), f1 = block_start(z1);
+                if (f1 - f0 > kSliceBlock) {
+                    // more than two blocks: only a smoothing length far beyond 2 h_init
+                    // set from outside reaches this far; report it instead of
+                    // summing in the wrong frame
+                    atomicOr(&w.st[0].bad, 8);
+                    continue;
+                }
                 R rb = reach_max;
                 if constexpr (HB) {
                     // the larger bound of h of the (at most two) blocks met
-                    const R hb0 = (R)w.cell_hmax[col + b0];
-                    const R hb1 = (b1 != b0) ? (R)w.cell_hmax[col + b1] : hb0;
+                    const R hb0 = (R)w.cell_hmax[col + f0];
+                    const R hb1 = (f1 != f0) ? (R)w.cell_hmax[col + f1] : hb0;
                     rb = reach_of(hb0 > hb1 ? hb0 : hb1);
                     const R rb2 = rb * rb;
                     if (gxy >= rb2) continue;
@@ -369,9 +377,9 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                 }
                 const uint32_t k0 = w.cstart[col + z0], k1 = w.cstart[col + z1 + 1];
                 if (k0 >= k1) continue;
-                const int f0 = block_start(z0), f1 = block_start(z1);
-                const uint32_t ks = (f1 != f0) ? w.cstart[col + f1] : k1;
-                fn(k0, k1, ks, (R)dx * wx, (R)dy * wy, (R)(f0 - fi) * wz, (R)(f1 - fi) * wz, rb);
+                const int g0 = block_start(z0), g1 = block_start(z1);
+                const uint32_t ks = (g1 != g0) ? w.cstart[col + g1] : k1;
+                fn(k0, k1, ks, (R)dx * wx, (R)dy * wy, (R)(g0 - fi) * wz, (R)(g1 - fi) * wz, rb);
             }
         }
         return;

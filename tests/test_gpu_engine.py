@@ -112,12 +112,26 @@ def test_checkpoint_restart_is_bit_exact(precision, gpu, tmp_path):
     assert dgot["t"] == dref["t"] and dgot["steps"] == dref["steps"] == 6
 
 
-def test_tuning_changes_speed_not_results_beyond_roundoff(gpu):
+@pytest.mark.parametrize("cell_factor", [1.0, 1.25, 3.0])
+def test_tuning_changes_speed_not_results_beyond_roundoff(cell_factor, gpu):
     ic = ics.tiny_impact()
     a, _ = run_states(ic, dsph.FP64, 4, True)
-    b, _ = run_states(ic, dsph.FP64, 4, True, dict(cell_factor=1.0, block_pair=64))
+    b, _ = run_states(ic, dsph.FP64, 4, True, dict(cell_factor=cell_factor, block_pair=64))
     for f in FIELDS:
         assert scale_err(b[f], a[f]) < 1e-12, f
+
+
+def test_tuning_limits_are_enforced(gpu):
+    """Columns narrower than h_init or a list skin above 0.25 could let a z range
+    reach past two slice blocks; the library refuses them."""
+    s = dsph.make_system(ics.tiny_impact(), dsph.MIXED)
+    for bad in (dict(cell_factor=0.5), dict(list_skin=0.5)):
+        with pytest.raises(dsph.DsphError) as e:
+            s.set_tuning(**bad)
+        assert e.value.code == -1
+    s.set_tuning(cell_factor=1.0, list_skin=0.25)
+    s.init()
+    s.close()
 
 
 @pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
