@@ -1,8 +1,8 @@
 // Device-side data layout of the SPH step.
 //
 // Particles live in structure-of-arrays form in HBM and are re-sorted every
-// step into cell order (Morton order inside 4x4x4-cell bricks, bricks in
-// row-major order).  Per-particle state that must survive a step ("carried"
+// step into cell order (sort columns across x and y, z slices within a
+// column, see below).  Per-particle state that must survive a step ("carried"
 // state) is double-buffered so the permutation is a pure gather; everything
 // recomputed each step lives in single buffers in the current order.
 #pragma once

@@ -3,11 +3,11 @@
 // strain-rate + momentum + energy pass.
 //
 // One lane owns one particle i of the cell-sorted order and walks the sort
-// cells within its own search radius (culling cells whose nearest point is out
-// of range), reading neighbour records from the cell table's contiguous
-// ranges.  Consecutive lanes hold spatially adjacent particles, so the
-// neighbour records a wave touches overlap heavily and are served from L1/L2.
-// Sums are gather-only (no atomics), so results do not depend on scheduling.
+// columns within its own search radius, reading in each one contiguous run of
+// neighbour records cut to its reach along z (see for_pieces).  Consecutive
+// lanes hold neighbouring particles of one column, so the neighbour records a
+// wave touches overlap heavily and are served from L1/L2.  Sums are
+// gather-only (no atomics), so results do not depend on scheduling.
 #pragma once
 #include "kernels_build.h"
 
@@ -580,8 +580,8 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
 
 // First density iteration of round 0, which also writes the neighbour lists:
 // while walking, every candidate j with
-//   r_ij < (1+skin)(h_i + max(h_i, hcell_j))
-// enters i's list in walk order (hcell_j: bound of h over j's sort cell at
+//   r_ij < (1+skin)(h_i + max(h_i, hblock_j))
+// enters i's list in walk order (hblock_j: bound of h over j's slice block at
 // the start of the step).  That covers r < 2h_i and r < h_i + h_j for as long
 // as no h grows by more than the skin.  Lists that do not fit are marked.
 template <class R, bool PER>
@@ -846,8 +846,8 @@ __device__ __forceinline__ bool force_list_usable(const Work<R>& w, int64_t i, i
     return lists && w.st[seg].list_bad == 0u && w.nbr_cnt[i] != kNbrOverflow;
 }
 
-// Force pass by walking the cells (radius h_i + max h, culled with the per-cell
-// bound of h).  With lists built, particles whose list is usable are left to
+// Force pass by walking the columns (radius h_i + max h, culled with the
+// per-block bound of h).  With lists built, particles whose list is usable are left to
 // k_force_list.
 template <class R, bool PER>
 __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, int lists, SegTargets<1> tg) {
