@@ -68,16 +68,26 @@ def step_flops(iterations: int = 2) -> int:
     return PAIRS_PER_PARTICLE * (iterations * FLOPS_DENSITY_PAIR + FLOPS_FORCE_PAIR)
 
 
+def _code_only(text: str) -> str:
+    """C++ source without comments and with whitespace runs collapsed (string
+    literals are kept as written)."""
+    import re
+    tok = re.compile(r'"(?:\\.|[^"\\])*"|//[^\n]*|/\*.*?\*/', re.S)
+    text = tok.sub(lambda m: m.group(0) if m.group(0).startswith('"') else " ", text)
+    return " ".join(text.split())
+
+
 def source_digest() -> str:
-    """Short digest of the device sources; ties a recorded counter profile to a build."""
+    """Short digest of the device sources' code (comments and layout ignored);
+    ties a recorded counter profile to a build."""
     import hashlib
     import os
     csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
     h = hashlib.sha256()
     for name in sorted(os.listdir(csrc)):
         if name.endswith((".h", ".hip")):
-            with open(os.path.join(csrc, name), "rb") as f:
-                h.update(name.encode() + b"\0" + f.read())
+            with open(os.path.join(csrc, name), encoding="utf-8") as f:
+                h.update(name.encode() + b"\0" + _code_only(f.read()).encode())
     return h.hexdigest()[:16]
 
 

@@ -170,3 +170,12 @@ def test_recorded_traffic_lines_are_matched_by_kernel_workload_and_digest(tmp_pa
     assert rec == {"bytes": 3.8e8, "fetch_factor": 1.9999, "write_factor": 1.0}
     assert roofline.recorded_traffic(str(tmp_path), "k_force<float, true>", "graze_1m") is None
     assert roofline.recorded_traffic(str(tmp_path), "k_force<float, false>", "layered_4m") is None
+
+
+def test_source_digest_ignores_comments_and_layout():
+    from deorbit_amd import roofline
+    a = "int f(int x) {  // doubles x\n    return 2 * x; /* done */\n}\n"
+    b = "/* header */\nint f(int x) {\n  return 2 * x;\n}"
+    c = "int f(int x) { return 3 * x; }"
+    assert roofline._code_only(a) == roofline._code_only(b) != roofline._code_only(c)
+    assert roofline._code_only('const char* s = "a // b";') == 'const char* s = "a // b";'
