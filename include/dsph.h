@@ -69,7 +69,7 @@ typedef struct {
 typedef struct {
     int32_t block_pair;         /* threads per workgroup, neighbour kernels      */
     int32_t block_point;        /* threads per workgroup, per-particle kernels   */
-    double cell_factor;         /* sort-column width = cell_factor * h_init (z slices: a third of it) */
+    double cell_factor;         /* sort-column width = cell_factor * h_init (z slices: a quarter of it) */
     int32_t xcd_remap;          /* 1 = map consecutive tiles onto one XCD        */
     int32_t cell_capacity_per_particle;  /* cell table size / particle count    */
     int32_t neighbor_capacity;  /* neighbour-list entries per particle; 0 = auto, <0 = no lists */
