@@ -154,7 +154,7 @@ class Engine : public EngineBase {
         w.h0 = dalloc<R>(n_);
         alloc_lists();
         // sort buffers
-        nb_sort_ = (int)blocks_for(n_, kRsTile);
+        nb_sort_ = (int)blocks_for(n_, kRsThreads * rs_items(n_));
         for (int k = 0; k < 2; k++) { keys_[k] = dalloc<uint32_t>(n_); vals_[k] = dalloc<uint32_t>(n_); }
         hist_ = dalloc<uint32_t>((size_t)256 * nb_sort_);
         digit_totals_ = dalloc<uint32_t>(256);
@@ -404,16 +404,17 @@ class Engine : public EngineBase {
         int ka = 0;
         for (int p = 0; p < passes_; p++) {
             const int shift = 8 * p;
-            launch(PH_SORT, "rs_hist", k_rs_hist, dim3(nb_sort_), dim3(kRsThreads), (const uint32_t*)keys_[ka], n_,
-                   shift, hist_, nb_sort_);
+            const bool small = rs_items(n_) == 8;
+            launch(PH_SORT, "rs_hist", small ? k_rs_hist<8> : k_rs_hist<16>, dim3(nb_sort_), dim3(kRsThreads),
+                   (const uint32_t*)keys_[ka], n_, shift, hist_, nb_sort_);
             launch(PH_SORT, "rs_scan", k_rs_scan, dim3(256), dim3(256), hist_, nb_sort_, digit_totals_);
-            launch(PH_SORT, "rs_scatter", k_rs_scatter, dim3(nb_sort_), dim3(kRsThreads), (const uint32_t*)keys_[ka],
-                   (const uint32_t*)vals_[ka], keys_[1 - ka], vals_[1 - ka], n_, shift, (const uint32_t*)hist_,
-                   nb_sort_, (const uint32_t*)digit_totals_);
+            launch(PH_SORT, "rs_scatter", small ? k_rs_scatter<8> : k_rs_scatter<16>, dim3(nb_sort_),
+                   dim3(kRsThreads), (const uint32_t*)keys_[ka], (const uint32_t*)vals_[ka], keys_[1 - ka],
+                   vals_[1 - ka], n_, shift, (const uint32_t*)hist_, nb_sort_, (const uint32_t*)digit_totals_);
             ka = 1 - ka;
         }
         // first slot of every cell (empty cells included), from the sorted keys
-        const unsigned ncs = std::min<unsigned>(blocks_for((int64_t)capacity_ + 1, 256), 2048u);
+        const unsigned ncs = std::min<unsigned>(blocks_for((int64_t)capacity_ + 1, 256 * kCellChunk), 2048u);
         launch(PH_SORT, "cell_start", k_cell_start, dim3(ncs), dim3(256), (const uint32_t*)keys_[ka], n_, w_.cstart,
                (const uint32_t*)d_total_cells_);
         const auto th = hmax_target(0);
@@ -460,14 +461,15 @@ class Engine : public EngineBase {
         for (int r = 0; r < 4; r++) {
             launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, r);
             const auto th = hmax_target(r + 1);
-            if (r == 0 && lists)
+            if (r == 0 && lists) {
                 launch(PH_DENSITY_BUILD, "density_build", k_density_build<R, PER>, dim3(nb), dim3(bq), cur(), w_,
                        tune_.xcd_remap);
-            else
-                launch(PH_DENSITY, "density", k_density<R, false, PER>, dim3(nb), dim3(bq), cur(), w_, r,
+                launch(PH_DENSITY, "density", k_density<R, true, PER>, dim3(nb), dim3(bq), cur(), w_, r,
                        tune_.xcd_remap, lists, th);
-            launch(PH_DENSITY, "density", k_density<R, true, PER>, dim3(nb), dim3(bq), cur(), w_, r,
-                   tune_.xcd_remap, lists, th);
+            } else {
+                launch(PH_DENSITY, "density", k_density_round<R, PER>, dim3(nb), dim3(bq), cur(), w_, r,
+                       tune_.xcd_remap, lists, th);
+            }
             fold_parts(PH_DENSITY, th, nb);
             if (r == 0 && lists) {
                 launch(PH_DENSITY_LIST, "density_list", k_density_list<R, PER>, dim3(nb), dim3(bq), cur(), w_,

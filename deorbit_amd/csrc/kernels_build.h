@@ -216,26 +216,34 @@ __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t*
 
 // First slot of every cell in the sorted keys (a lower bound per cell, so empty
 // cells get the slot where they would start), for cells 0 .. total inclusive.
+// A thread takes kCellChunk consecutive cells: one binary search for the first,
+// then a merge walk along the keys for the rest.
+constexpr int kCellChunk = 8;
 __global__ __launch_bounds__(256) void k_cell_start(const uint32_t* skeys, int64_t n, uint32_t* cstart,
                                                    const uint32_t* total_cells) {
-    const uint32_t total = *total_cells;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t c = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); c <= total; c += stride) {
+    const uint64_t total = *total_cells;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kCellChunk;
+    for (uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kCellChunk; c0 <= total; c0 += stride) {
         int64_t lo = 0, hi = n;
         while (lo < hi) {
             const int64_t mid = (lo + hi) >> 1;
-            if (skeys[mid] < c) lo = mid + 1;
+            if (skeys[mid] < c0) lo = mid + 1;
             else hi = mid;
         }
-        cstart[c] = (uint32_t)lo;
+        const uint64_t c1 = c0 + kCellChunk < total + 1 ? c0 + kCellChunk : total + 1;
+        for (uint64_t c = c0; c < c1; c++) {
+            while (lo < n && skeys[lo] < c) lo++;
+            cstart[c] = (uint32_t)lo;
+        }
     }
 }
 
 // ---------------------------------------------------------------- radix sort
-// LSD radix sort of (key, value) pairs, 8-bit digits, 4096-key tiles.
+// LSD radix sort of (key, value) pairs, 8-bit digits, tiles of ITEMS keys per
+// thread (rs_items picks 8 below 2M keys, so that a pass still has two tiles
+// per CU, else 16).
 constexpr int kRsThreads = 256;
-constexpr int kRsItems = 16;
-constexpr int kRsTile = kRsThreads * kRsItems;
+constexpr int rs_items(int64_t n) { return n < (int64_t)2000000 ? 8 : 16; }
 
 // Lanes of the wave holding the same 8-bit digit as this lane.
 __device__ __forceinline__ unsigned long long match_digit(uint32_t d, bool valid) {
@@ -249,17 +257,25 @@ __device__ __forceinline__ unsigned long long match_digit(uint32_t d, bool valid
     return m;
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* keys, int64_t n, int shift,
                                                        uint32_t* hist, int nb) {
     __shared__ uint32_t h[256];
     const int t = threadIdx.x, lane = lane_id();
     h[t] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kRsTile;
-    for (int k = 0; k < kRsItems; k++) {
+    const int64_t base = (int64_t)blockIdx.x * (kRsThreads * ITEMS);
+    // the whole tile's keys are loaded first, so their loads overlap
+    uint32_t kk[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
         const int64_t idx = base + (int64_t)k * kRsThreads + t;
-        const bool valid = idx < n;
-        const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+        kk[k] = idx < n ? keys[idx] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const bool valid = base + (int64_t)k * kRsThreads + t < n;
+        const uint32_t d = (kk[k] >> shift) & 255u;
         const unsigned long long m = match_digit(d, valid);
         if (valid && __builtin_ctzll(m) == lane) atomicAdd(&h[d], (uint32_t)__popcll(m));
     }
@@ -294,6 +310,7 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* hist, int nb, uint32_
     if (t == 255) totals[blockIdx.x] = part[255];
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin,
                                                           uint32_t* kout, uint32_t* vout, int64_t n,
                                                           int shift, const uint32_t* hist, int nb,
@@ -317,12 +334,20 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, 
     base[t] = dbase + hist[(size_t)t * nb + blockIdx.x];
     for (int q = 0; q < 4; q++) wc[q][t] = 0;
     __syncthreads();
-    const int64_t tile = (int64_t)blockIdx.x * kRsTile;
-    for (int k = 0; k < kRsItems; k++) {
+    const int64_t tile = (int64_t)blockIdx.x * (kRsThreads * ITEMS);
+    // the whole tile is loaded first, so its loads overlap
+    uint32_t kk[ITEMS], vv[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
         const int64_t idx = tile + (int64_t)k * kRsThreads + t;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t val = valid ? vin[idx] : 0u;
+        kk[k] = idx < n ? kin[idx] : 0u;
+        vv[k] = idx < n ? vin[idx] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const bool valid = tile + (int64_t)k * kRsThreads + t < n;
+        const uint32_t key = kk[k];
+        const uint32_t val = vv[k];
         const uint32_t d = (key >> shift) & 255u;
         const unsigned long long m = match_digit(d, valid);
         const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));

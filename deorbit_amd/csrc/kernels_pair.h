@@ -530,6 +530,50 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
     }
 }
 
+// Both iterations of density round r in one pass (sph.hpp:189-201): the sum of
+// particle i uses only its own h, and both iterations of a round share the
+// round's reference grid, so the second iteration needs nothing but the h the
+// first one gives i.  Same arithmetic as k_density run twice; the first
+// iteration's rho and h stay in registers.  Used for every round except round
+// 0 with lists, whose iterations build and read the lists.
+template <class R, bool PER>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists,
+                                                       SegTargets<1> tg) {
+    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    int seg = 0;
+    bool act = false;
+    if (i < w.n) {
+        seg = info_seg(c.info[i]);
+        act = w.st[seg].active[r] != 0;
+    }
+    double hnew_d = 0.0;
+    if (act) {
+        const SegPar& p = w.par[seg];
+        const SegGrid& g = w.grid[seg];
+        DensLane<R, PER> L;
+        const uint32_t cc = w.cellc[i];
+        L.me = w.dens_rec[i];
+        own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
+        const bool finite = isfinite((double)L.ox) && isfinite((double)L.oy) && isfinite((double)L.oz);
+        L.h = c.h[i];
+        R rho = rho_floor<R>();
+        if (finite) {
+            L.setup(c, p, g, i, cc);
+            rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
+        }
+        L.h = density_h<R>(c, p, i, rho);
+        rho = rho_floor<R>();
+        if (finite) {
+            L.setup(c, p, g, i, cc);
+            rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
+        }
+        hnew_d = (double)density_store<R, true>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
+    }
+    const double v[1] = {hnew_d};
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
+}
+
 // Second density iteration of round 0 for the particles whose list is usable:
 // the list holds the walk's pairs in the walk's order, so the sum is the one
 // the walk would give.  Four entries are read ahead so their loads overlap.
