@@ -310,31 +310,51 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* hist, int nb, uint32_
     if (t == 255) totals[blockIdx.x] = part[255];
 }
 
+// Stable scatter of one tile: keys are ranked by digit (wave ballots, then
+// waves in order, then rounds in order), placed in LDS in digit order, and
+// written out so that consecutive threads write consecutive slots of a
+// digit's run.
 template <int ITEMS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin,
                                                           uint32_t* kout, uint32_t* vout, int64_t n,
                                                           int shift, const uint32_t* hist, int nb,
                                                           const uint32_t* totals) {
-    __shared__ uint32_t base[256];
+    constexpr int kTile = kRsThreads * ITEMS;
+    __shared__ uint32_t gofs[256];    // global slot of digit d's run in this tile minus its local slot
+    __shared__ uint32_t lbase[256];   // next local slot of digit d
     __shared__ uint32_t wc[4][256];
     __shared__ uint32_t pre[4][256];
+    __shared__ uint32_t sk[kTile], sv[kTile];
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     // digit base = exclusive scan of the digit totals (each block redoes this 256-entry scan)
     const uint32_t tot = totals[t];
-    base[t] = tot;
+    gofs[t] = tot;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t v = (t >= off) ? base[t - off] : 0u;
+        const uint32_t v = (t >= off) ? gofs[t - off] : 0u;
         __syncthreads();
-        base[t] += v;
+        gofs[t] += v;
         __syncthreads();
     }
-    const uint32_t dbase = base[t] - tot;
+    const uint32_t dbase = gofs[t] - tot;
+    // this tile's keys of digit t (the scanned histogram row holds the earlier tiles')
+    const uint32_t h0 = hist[(size_t)t * nb + blockIdx.x];
+    const uint32_t cnt = ((int)blockIdx.x + 1 < nb ? hist[(size_t)t * nb + blockIdx.x + 1] : tot) - h0;
+    lbase[t] = cnt;
     __syncthreads();
-    base[t] = dbase + hist[(size_t)t * nb + blockIdx.x];
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = (t >= off) ? lbase[t - off] : 0u;
+        __syncthreads();
+        lbase[t] += v;
+        __syncthreads();
+    }
+    const uint32_t lstart = lbase[t] - cnt;
+    __syncthreads();
+    lbase[t] = lstart;
+    gofs[t] = dbase + h0 - lstart;    // modulo 2^32; gofs + local slot is the global slot
     for (int q = 0; q < 4; q++) wc[q][t] = 0;
     __syncthreads();
-    const int64_t tile = (int64_t)blockIdx.x * (kRsThreads * ITEMS);
+    const int64_t tile = (int64_t)blockIdx.x * kTile;
     // the whole tile is loaded first, so its loads overlap
     uint32_t kk[ITEMS], vv[ITEMS];
 #pragma unroll
@@ -347,7 +367,6 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, 
     for (int k = 0; k < ITEMS; k++) {
         const bool valid = tile + (int64_t)k * kRsThreads + t < n;
         const uint32_t key = kk[k];
-        const uint32_t val = vv[k];
         const uint32_t d = (key >> shift) & 255u;
         const unsigned long long m = match_digit(d, valid);
         const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -360,14 +379,26 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* kin, 
         pre[3][t] = c0 + c1 + c2;
         __syncthreads();
         if (valid) {
-            const uint32_t pos = base[d] + pre[wave][d] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
+            const uint32_t lp = lbase[d] + pre[wave][d] + rank;
+            sk[lp] = key;
+            sv[lp] = vv[k];
         }
         __syncthreads();
-        base[t] += c0 + c1 + c2 + c3;
+        lbase[t] += c0 + c1 + c2 + c3;
         for (int q = 0; q < 4; q++) wc[q][t] = 0;
         __syncthreads();
+    }
+    const int64_t left = n - tile;
+    const int count = left < kTile ? (int)left : kTile;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const int idx = k * kRsThreads + t;
+        if (idx < count) {
+            const uint32_t key = sk[idx];
+            const uint32_t pos = gofs[(key >> shift) & 255u] + (uint32_t)idx;
+            kout[pos] = key;
+            vout[pos] = sv[idx];
+        }
     }
 }
 
