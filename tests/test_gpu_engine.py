@@ -5,6 +5,7 @@ import pytest
 
 from conftest import scale_err
 from deorbit_amd import dsph, formats, ics
+from deorbit_amd import materials as M
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -200,6 +201,27 @@ def test_full_size_fp64_init_matches_oracle_graze_1m(gpu):
         print(f"graze_1m init fp64 {name}={e:.2e}", flush=True)
         assert e < 1e-12, name
     assert abs(s.timestep() - o.timestep()) <= 1e-14 * o.timestep()
+
+
+def test_init_above_two_million_particles_matches_oracle(gpu):
+    """2.08M particles (R=30dx at 30 deg into a 160x96x128 slab): the sort uses its
+    16-key tiles from 2M keys on, so the cell list of this size is checked here
+    against the oracle's init, in FP64 to 1e-12 and in mixed precision to the FP32
+    density bar."""
+    ic = ics.impact(30, 30.0, 8000.0, 160, 96, [(M.MAT_TARGET, 128)], seed=2001, strength=True, frac=True,
+                    name="impact_2m", steps=10)
+    assert len(ic["mass"]) > 2_000_000
+    o = po.make_system(ic, ic["params"], ic["materials"], seed_damage=12345, dx=ic["dx"])
+    o.init()
+    for precision, bar in ((dsph.FP64, 1e-12), (dsph.MIXED, 1e-5)):
+        s = dsph.make_system(ic, precision)
+        s.init()
+        fields = (("rho", po.RHO), ("hh", po.HH), ("P", po.P), ("acc", po.ACC))
+        for name, f in fields[:2] if precision == dsph.MIXED else fields:
+            e = scale_err(s.get(name), o.get(f))
+            print(f"impact_2m init {'fp64' if precision == dsph.FP64 else 'mixed'} {name}={e:.2e}", flush=True)
+            assert e < bar, (precision, name)
+        s.close()
 
 
 def test_energy_rate_balance_impact_37k_mixed(gpu):
