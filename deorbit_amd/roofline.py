@@ -28,10 +28,10 @@ def kernel_bytes(precision: str = "mixed") -> dict:
     out["keys"] = 4 + 3 * d + 8
     # radix sort, per pass: hist reads keys; scatter reads and writes key+value
     out["sort_pass"] = 4 + 8 + 8
-    # permute: carried state R+W (8 doubles, h D eps_act, S6, info, orig) + keys/vals R
-    #          + old derivatives R+W (dudt, dSdt6) + density record W + cellc W
+    # permute: carried state R+W (8 doubles, h D eps_act, S6, info, orig) + keys/vals/dev_of R
+    #          + perm W + old derivatives R+W (dudt, dSdt6) + density record W + cellc, h0 W
     carried = 8 * d + 3 * r + 6 * r + 4 + 4
-    out["permute"] = 2 * carried + 8 + 2 * 7 * r + rec + 4
+    out["permute"] = 2 * carried + 12 + 4 + 2 * 7 * r + rec + 4 + r
     # density iteration: neighbour record once + own (h, cellc, info, mass) R; W rho, h
     out["density"] = rec + r + 4 + 4 + d + 2 * r
     # last iteration adds the EOS epilogue: R u, S6, D, vel; W P cs csig Pf dscale Rart sigmax + 5 records
