@@ -535,10 +535,12 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
 // round's reference grid, so the second iteration needs nothing but the h the
 // first one gives i.  Same arithmetic as k_density run twice; the first
 // iteration's rho and h stay in registers.  Used for every round except round
-// 0 with lists, whose iterations build and read the lists.
+// 0 with lists, whose iterations build and read the lists.  The FP32 variant is
+// held at five waves per SIMD (96 VGPRs, a few epilogue values spilled): at its
+// natural four waves the walk is slower.
 template <class R, bool PER>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists,
-                                                       SegTargets<1> tg) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
+void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
     int seg = 0;
