@@ -914,7 +914,10 @@ class Engine : public EngineBase {
         dsph_tuning best = tune_;
         double best_ms = 1e300;
         const int blocks[3] = {64, 128, 256};
-        const double cells[4] = {1.0, 1.25, 1.5, 2.0};
+        // column widths in h_init; on lattice initial conditions the step time
+        // varies sharply around 1.5 (how lattice lines fall into columns), hence
+        // the close neighbours
+        const double cells[6] = {1.0, 1.25, 1.45, 1.5, 1.55, 2.0};
         const int caps[2] = {-1, 128};          // cell walks, neighbour lists
         hipEvent_t e0, e1;
         HIPC(hipEventCreate(&e0));
