@@ -167,6 +167,24 @@ __device__ __forceinline__ double wave_min_d(double v) {
     return v;
 }
 
+// atomicMax of the non-negative float v at p[key] over the valid lanes of the
+// wave, one atomic per distinct key (neighbouring lanes mostly share a key, and
+// same-address atomics from one wave serialise).  Every lane of the wave must
+// call this.
+__device__ __forceinline__ void wave_max_at(float* p, uint32_t key, float v, bool valid) {
+    unsigned long long pending = __ballot(valid);
+    const int lane = lane_id();
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const uint32_t lkey = (uint32_t)__shfl((int)key, leader);
+        const bool mine = valid && key == lkey;
+        float x = mine ? v : 0.0f;
+        for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+        if (lane == leader) atomicMax((unsigned int*)&p[lkey], __float_as_uint(x));
+        pending &= ~__ballot(mine);
+    }
+}
+
 // Reduce `v` over the lanes of each segment present in the wave and apply one
 // atomic per (wave, segment).  Every lane of the wave must call this.
 // kind: 0 max, 1 min.  Values are encoded with `enc` (monotone uint64 map).

@@ -193,20 +193,24 @@ __device__ __forceinline__ void eos_epilogue(const Carried<R>& c, const Work<R>&
     const R sc = dsc * ir2;
     sa.x = s6[0] * sc; sa.y = s6[1] * sc; sa.z = s6[2] * sc; sa.w = s6[3] * sc;
     sb.x = s6[4] * sc; sb.y = s6[5] * sc; sb.z = mi / rho; sb.w = R(0);
-    // per-block upper bound of h for the force stencil (positive floats order as integers)
-    const uint32_t cc = w.cellc[i];
-    const SegGrid& g = w.grid[info_seg(c.info[i])];
-    if (isfinite((double)ox)) {
-        float hf = (float)h;
-        if ((double)hf < (double)h) hf = __uint_as_float(__float_as_uint(hf) + 1u);   // round up
-        atomicMax((unsigned int*)&w.cell_hmax[block_key(g, cc & 1023u, (cc >> 10) & 1023u, cc >> 20)],
-                  __float_as_uint(hf));
-    }
     w.frec_pos[i] = a;
     w.frec_vel[i] = b;
     w.frec_th[i] = t;
     w.frec_sa[i] = sa;
     w.frec_sb[i] = sb;
+}
+
+// Raise the upper bound of h of particle i's slice block, which the force
+// stencil culls with, to its final h (rounded up to float; positive floats
+// order as integers), for the valid lanes.  Called after the EOS epilogue by
+// every lane of the wave: one atomic per block the wave touches.
+template <class R>
+__device__ __forceinline__ void raise_h_bound(const Work<R>& w, const SegGrid& g, uint32_t cc, double h,
+                                              bool valid) {
+    const int cx = cc & 1023u, cy = (cc >> 10) & 1023u, cz = cc >> 20;
+    float hf = (float)h;
+    if ((double)hf < h) hf = __uint_as_float(__float_as_uint(hf) + 1u);
+    wave_max_at(w.cell_hmax, valid ? block_key(g, cx, cy, cz) : 0u, hf, valid);
 }
 
 // ---------------------------------------------------------------- neighbour lists
@@ -503,6 +507,8 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
     }
     const bool by_list = LAST && lists && r == 0;
     double hnew_d = 0.0;
+    uint32_t cc = 0;
+    bool fin = false;
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
@@ -512,7 +518,7 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
             act = false;                           // k_density_list owns this particle
         } else {
             if (by_list) w.nbr_cnt[i] = kNbrOverflow;
-            const uint32_t cc = w.cellc[i];
+            cc = w.cellc[i];
             L.me = w.dens_rec[i];
             own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
             const bool finite = isfinite((double)L.ox) && isfinite((double)L.oy) && isfinite((double)L.oz);
@@ -522,9 +528,11 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
                 rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
             }
             hnew_d = (double)density_store<R, LAST>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
+            fin = finite;
         }
     }
     if (LAST) {
+        raise_h_bound<R>(w, w.grid[seg], cc, hnew_d, act && fin);
         const double v[1] = {hnew_d};
         block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
     }
@@ -550,14 +558,17 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
         act = w.st[seg].active[r] != 0;
     }
     double hnew_d = 0.0;
+    uint32_t cc = 0;
+    bool fin = false;
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
         DensLane<R, PER> L;
-        const uint32_t cc = w.cellc[i];
+        cc = w.cellc[i];
         L.me = w.dens_rec[i];
         own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
         const bool finite = isfinite((double)L.ox) && isfinite((double)L.oy) && isfinite((double)L.oz);
+        fin = finite;
         L.h = c.h[i];
         R rho = rho_floor<R>();
         if (finite) {
@@ -572,6 +583,7 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
         }
         hnew_d = (double)density_store<R, true>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
     }
+    raise_h_bound<R>(w, w.grid[seg], cc, hnew_d, act && fin);
     const double v[1] = {hnew_d};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
 }
@@ -591,14 +603,17 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
         act = w.st[seg].active[0] != 0 && dens_list_usable<R>(w, i, c.h[i]);
     }
     double hnew_d = 0.0;
+    uint32_t cc = 0;
+    bool fin = false;
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
         DensLane<R, PER> L;
         L.h = c.h[i];
-        const uint32_t cc = w.cellc[i];
+        cc = w.cellc[i];
         L.me = w.dens_rec[i];
         own_offset<R>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
+        fin = isfinite((double)L.ox);
         L.setup(c, p, g, i, cc);
         const uint32_t nc = w.nbr_cnt[i];
         const int cap = w.nbr_cap;
@@ -620,6 +635,7 @@ __global__ __launch_bounds__(256) void k_density_list(Carried<R> c, Work<R> w, i
         const R rho = smax(s, rho_floor<R>());
         hnew_d = (double)density_store<R, true>(c, w, i, seg, p, rho, true, 1, L.ox, L.oy, L.oz);
     }
+    raise_h_bound<R>(w, w.grid[seg], cc, hnew_d, act && fin);
     const double v[1] = {hnew_d};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
 }
@@ -710,13 +726,23 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
 template <class R>
 __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w.n) return;
-    const uint32_t inf = c.info[i];
-    const int seg = info_seg(inf);
-    const SegGrid& g = w.grid[seg];
-    R ox, oy, oz;
-    own_offset<R>(c, g, i, w.cellc[i], w.dens_rec[i], ox, oy, oz);
-    eos_epilogue<R>(c, w, i, w.par[seg], info_mat(inf), w.rho[i], c.h[i], ox, oy, oz);
+    const bool in = i < w.n;
+    int seg = 0;
+    uint32_t cc = 0;
+    double h = 0.0;
+    bool fin = false;
+    if (in) {
+        const uint32_t inf = c.info[i];
+        seg = info_seg(inf);
+        const SegGrid& g = w.grid[seg];
+        cc = w.cellc[i];
+        R ox, oy, oz;
+        own_offset<R>(c, g, i, cc, w.dens_rec[i], ox, oy, oz);
+        eos_epilogue<R>(c, w, i, w.par[seg], info_mat(inf), w.rho[i], c.h[i], ox, oy, oz);
+        h = (double)c.h[i];
+        fin = isfinite((double)ox);
+    }
+    raise_h_bound<R>(w, w.grid[seg], cc, h, fin);
 }
 
 // ---------------------------------------------------------------- forces
