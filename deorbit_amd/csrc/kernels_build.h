@@ -419,6 +419,9 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
     const bool in = i < n;
     int seg = 0;
     double hv = 0.0;
+    uint32_t bkey = 0;
+    float bh = 0.0f;
+    bool bvalid = false;
     if (in) {
         const uint32_t s = w.dev_of[vals[i]];   // current slot of the particle sorted to i
         w.perm[i] = s;
@@ -445,12 +448,9 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
         w.h0[i] = h;
         if (key != g.cell_base + g.ncells - 1u) {
             const int cx = cell_coord(g, 0, x), cy = cell_coord(g, 1, y), cz = cell_coord(g, 2, z);
-            // per-block bound of h at the start of the step, read only by the
-            // neighbour-list build (the density epilogue raises it to the final
-            // h that the force pass culls with)
-            if (list_bounds)
-                atomicMax((unsigned int*)&w.cell_hmax[block_key(g, cx, cy, cz)],
-                          __float_as_uint(float_at_least((double)h)));
+            bkey = block_key(g, cx, cy, cz);
+            bh = float_at_least((double)h);
+            bvalid = true;
             w.cellc[i] = cellc_pack(cx, cy, cz);
             V4 rec;
             if (sizeof(R) == 4) {
@@ -471,6 +471,10 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
         }
         hv = (double)h;
     }
+    // per-block bound of h at the start of the step, read only by the
+    // neighbour-list build (the density epilogue raises it to the final h that
+    // the force pass culls with)
+    if (list_bounds) wave_max_at(w.cell_hmax, bkey, bh, bvalid);
     const double v[1] = {hv};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, in);
 }
