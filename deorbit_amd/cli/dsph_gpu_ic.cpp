@@ -333,8 +333,13 @@ int main(int argc, char** argv) {
     try {
         while (t < t_end && (max_steps < 0 || step < max_steps)) {
             // device CFL step (sph.hpp:350-355), clamped to land on t_end (gpu_ic.cpp:130)
-            const double dtc = S.timestep();
+            const double dtc = S.timestep();     // raises the device's trouble flags
+            if (!(dtc > 0.0) || !std::isfinite(dtc)) {
+                std::printf("step %ld failed: device time step %g is not positive and finite\n", step, dtc);
+                return 1;
+            }
             const double a = now();
+            const double t_prev = t;
             if (t + dtc > t_end) {
                 S.step(t_end - t);
                 t = t_end;
@@ -345,10 +350,14 @@ int main(int argc, char** argv) {
                 const double horizon = std::min(std::min(t_end, next_snap), next_ckpt) - t;
                 if (dtc > 0 && horizon > 4 * dtc) k = std::max(1, std::min(graph_steps, (int)(horizon / dtc) - 2));
                 if (max_steps >= 0) k = (int)std::min<long>(k, max_steps - step);
-                S.run(k, 0.0, true);
+                S.run(k, 0.0, true);             // raises the device's trouble flags
                 dsph_diag d = S.diagnostics();
                 t = d.t;
                 step += k;
+            }
+            if (!(t > t_prev)) {
+                std::printf("step %ld failed: the clock did not advance (t=%.9g)\n", step, t);
+                return 1;
             }
             T_step += now() - a;
             if (step / 25 != last_report / 25 || t >= t_end) { report(t); last_report = step; }

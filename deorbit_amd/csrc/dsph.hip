@@ -625,15 +625,27 @@ class Engine : public EngineBase {
         release_graphs();
     }
 
+    // The device's trouble flags of one segment as a status and a message (the
+    // flags are sticky: once set, every later call of the context reports them).
+    static int seg_status(const SegState& s, int seg, std::string* msg) {
+        const std::string where = " (segment " + std::to_string(seg) + ")";
+        if (s.bad & 1) { if (msg) *msg = "cell table capacity exceeded"; return DSPH_E_RANGE; }
+        if (s.bad & 4) { if (msg) *msg = "non-finite particle position" + where; return DSPH_E_NUMERIC; }
+        if (s.bad & 2) { if (msg) *msg = "non-finite or non-positive time step" + where; return DSPH_E_NUMERIC; }
+        if (s.bad & 8) {
+            if (msg) *msg = "a smoothing length far above 2 h_init reaches past two slice blocks; "
+                            "pairs beyond them were not summed" + where;
+            return DSPH_E_RANGE;
+        }
+        return DSPH_OK;
+    }
+    // Synchronises the stream, then raises the first segment's trouble flags.
     int check_health() {
         auto st = read_state();
         for (int s = 0; s < nseg_; s++) {
-            if (st[s].bad & 1) throw Err{DSPH_E_RANGE, "cell table capacity exceeded"};
-            if (st[s].bad & 4) throw Err{DSPH_E_NUMERIC, "non-finite particle position in segment " + std::to_string(s)};
-            if (st[s].bad & 2) throw Err{DSPH_E_NUMERIC, "non-finite or negative time step in segment " + std::to_string(s)};
-            if (st[s].bad & 8)
-                throw Err{DSPH_E_RANGE, "a smoothing length far above 2 h_init reaches past two slice blocks; "
-                                        "use a larger cell_factor"};
+            std::string msg;
+            const int rc = seg_status(st[s], s, &msg);
+            if (rc != DSPH_OK) throw Err{rc, msg};
         }
         return 0;
     }
@@ -642,6 +654,7 @@ class Engine : public EngineBase {
         if (!have_dt_) throw Err{DSPH_E_STATE, "no force evaluation yet: call dsph_init first"};
         auto st = read_state();
         for (int s = 0; s < nseg_; s++) out[s] = bits_to_double(st[s].dt_next);
+        check_health();
     }
 
     // ---------------------------------------------------------------- fields
@@ -808,6 +821,7 @@ class Engine : public EngineBase {
         out->density_rounds = s.rounds;
         out->list_walk_steps = (int64_t)s.list_walk_steps;
         out->list_overflow = (int64_t)s.list_overflow;
+        out->status = seg_status(s, seg, nullptr);
     }
 
     // ---------------------------------------------------------------- profile / tuning
@@ -1256,17 +1270,17 @@ int dsph_timestep(dsph_ctx* c, double* dt_out) {
     CTX_OP(require(dt_out != nullptr, DSPH_E_ARG, "null output"); e.timestep(dt_out));
 }
 int dsph_step(dsph_ctx* c, double dt) {
-    CTX_OP(require(std::isfinite(dt), DSPH_E_ARG, "dt must be finite"); e.step_host(dt));
+    CTX_OP(require(std::isfinite(dt), DSPH_E_ARG, "dt must be finite"); e.step_host(dt); e.check_health());
 }
 int dsph_step_dts(dsph_ctx* c, const double* dts) {
     CTX_OP(require(dts != nullptr, DSPH_E_ARG, "null dts");
            for (size_t s = 0; s < c->segp.size(); s++) require(dts[s] >= 0 && std::isfinite(dts[s]), DSPH_E_ARG,
                                                                   "per-segment dt must be finite and non-negative");
-           e.step_dts(dts));
+           e.step_dts(dts); e.check_health());
 }
 int dsph_run(dsph_ctx* c, int32_t nsteps, double dt, int32_t use_graph) {
     CTX_OP(require(nsteps >= 0 && std::isfinite(dt), DSPH_E_ARG, "nsteps must be >= 0 and dt finite");
-           e.run(nsteps, dt, use_graph != 0));
+           e.run(nsteps, dt, use_graph != 0); e.check_health());
 }
 int dsph_synchronize(dsph_ctx* c) { CTX_OP(e.sync(); e.check_health()); }
 int dsph_set_clock(dsph_ctx* c, int32_t seg, double t, int64_t steps, double dt_next) {

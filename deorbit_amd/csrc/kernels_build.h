@@ -28,8 +28,12 @@ __global__ void k_step_begin(SegState* st, const SegPar* par, int nseg, int mode
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
         SegState& S = st[s];
         double dt = (mode == 0) ? bits_to_double(S.dt_next) : (mode == 1 ? host_dt : seg_dt[s]);
-        // adaptive runs stop exactly at t_end (run_ic.cpp:327-330)
-        if (mode == 0 && par[s].t_end > 0.0 && S.t + dt > par[s].t_end) dt = par[s].t_end - S.t;
+        // a CFL step must be positive: zero (an infinite signal speed) would
+        // stall the segment's clock
+        if (mode == 0 && !(dt > 0.0)) S.bad |= 2;
+        // adaptive runs stop exactly at t_end (run_ic.cpp:327-330); a finished
+        // segment takes steps of zero
+        if (mode == 0 && par[s].t_end > 0.0 && S.t + dt > par[s].t_end) dt = fmax(par[s].t_end - S.t, 0.0);
         if (!(dt > 0.0)) {
             if (!(dt == 0.0)) S.bad |= 2;
             dt = 0.0;

@@ -361,7 +361,7 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                     // more than two blocks: only a smoothing length far beyond 2 h_init
                     // set from outside reaches this far; report it instead of
                     // summing in the wrong frame
-                    atomicOr(&w.st[0].bad, 8);
+                    atomicOr(&w.st[&g - w.grid].bad, 8);
                     continue;
                 }
                 R rb = reach_max;

@@ -67,7 +67,8 @@ class Diag(C.Structure):
                 ("hmax", C.c_double), ("rho_min", C.c_double), ("rho_max", C.c_double),
                 ("dmax", C.c_double), ("t", C.c_double), ("dt_next", C.c_double),
                 ("steps", C.c_int64), ("density_rounds", C.c_int32), ("nonfinite", C.c_int32),
-                ("list_walk_steps", C.c_int64), ("list_overflow", C.c_int64)]
+                ("list_walk_steps", C.c_int64), ("list_overflow", C.c_int64),
+                ("status", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Profile(C.Structure):

@@ -12,6 +12,13 @@
  *     {xx, yy, zz, xy, xz, yz} (the Sym3 order of sph.hpp:31-33);
  *   - every call returns 0 or a negative dsph_status; the text of the last
  *     error of the calling thread is dsph_last_error();
+ *   - the kernels record trouble in per-segment device flags (cell table
+ *     overflow, a non-finite position, a non-positive or non-finite time step,
+ *     a smoothing length whose reach exceeds the cell layout).  The flags are
+ *     sticky; every call that synchronises with the device (dsph_init, the
+ *     phase methods, dsph_step, dsph_step_dts, dsph_run, dsph_timestep,
+ *     dsph_synchronize) returns them as DSPH_E_RANGE or DSPH_E_NUMERIC, and
+ *     dsph_diagnostics reports them per segment in dsph_diag.status;
  *   - a context is bound to one HIP device and one HIP stream; it is
  *     thread-compatible, not thread-safe.
  * No C++ exceptions cross this boundary.
@@ -23,7 +30,7 @@
 extern "C" {
 #endif
 
-#define DSPH_ABI_VERSION 1
+#define DSPH_ABI_VERSION 2
 
 typedef struct dsph_ctx dsph_ctx;
 
@@ -89,6 +96,8 @@ typedef struct {
     int32_t nonfinite;          /* count of non-finite pos/vel/u values         */
     int64_t list_walk_steps;    /* force passes that walked cells: an h outgrew the list skin */
     int64_t list_overflow;      /* particle neighbour lists that did not fit (cumulative)    */
+    int32_t status;             /* DSPH_OK, or the dsph_status of the segment's device flags  */
+    int32_t pad_;
 } dsph_diag;
 
 /* Per-phase device time from HIP events (profiling mode). */
@@ -148,13 +157,15 @@ int dsph_grow_damage(dsph_ctx* ctx, double dt);       /* System::grow_damage    
 /* System::timestep; dt_out holds one value per segment. */
 int dsph_timestep(dsph_ctx* ctx, double* dt_out);
 /* System::step(dt). dt > 0: every segment steps by dt.  dt <= 0: each
- * segment uses its own device CFL step (no host round trip). */
+ * segment uses its own device CFL step (no host round trip for the dt).
+ * Waits for the step and returns the device flags. */
 int dsph_step(dsph_ctx* ctx, double dt);
 /* Per-segment host-chosen steps (nseg values). */
 int dsph_step_dts(dsph_ctx* ctx, const double* dts);
 /* nsteps steps with device-resident CFL dt (dt <= 0) or a fixed dt, replayed
  * from a captured hipGraph when use_graph != 0.  Replaces the host loop of
- * gpu_ic.cpp:128-144 without its per-step host reads. */
+ * gpu_ic.cpp:128-144 without its per-step host reads.  Waits once, after the
+ * last step, and returns the device flags. */
 int dsph_run(dsph_ctx* ctx, int32_t nsteps, double dt, int32_t use_graph);
 int dsph_synchronize(dsph_ctx* ctx);
 /* Set the clock of segment seg (< 0: all): simulated time, step count and the

@@ -5,7 +5,11 @@
 // seed_damage, init, timestep, step and the phase methods), so a driver
 // written against that interface compiles against this one.  Field access is
 // by whole-array copies in insertion order instead of public std::vectors,
-// because the state lives in device memory.  Errors throw dsph::Error.
+// because the state lives in device memory.  Errors throw dsph::Error: argument
+// and state errors at once, and the device's trouble flags (non-finite state,
+// a non-positive time step, capacity and reach limits) from every call that
+// waits for the device (init, the phase methods, step, run, timestep and
+// synchronize).
 #pragma once
 #include <stdexcept>
 #include <string>
@@ -97,6 +101,9 @@ class System {
     double timestep() { flush(); double dt = 0; check(dsph_timestep(ctx_, &dt)); return dt; }
     void step(double dt) { flush(); check(dsph_step(ctx_, dt)); }
     void run(int nsteps, double dt = 0.0, bool graph = true) { flush(); check(dsph_run(ctx_, nsteps, dt, graph)); }
+    // wait for the device and raise its trouble flags (no reference counterpart:
+    // the reference's calls return synchronously)
+    void synchronize() { flush(); check(dsph_synchronize(ctx_)); }
     dsph_diag diagnostics() { flush(); dsph_diag d{}; check(dsph_diagnostics(ctx_, 0, &d)); return d; }
 
     // whole-field copies in insertion order (see dsph_field for widths and types)

@@ -265,12 +265,46 @@ def test_nonfinite_particle_is_parked_and_reported(gpu):
     pos = s.get("pos")
     pos[5] = np.nan
     s.set("pos", pos)
-    s.step(1e-6)                       # must finish: the particle sits in its segment's park cell
+    # the step finishes (the particle sits in its segment's park cell) and
+    # reports the device's flag
     with pytest.raises(dsph.DsphError) as e:
+        s.step(1e-6)
+    assert e.value.code == -6 and "non-finite particle position" in str(e.value)
+    with pytest.raises(dsph.DsphError) as e:          # the flag is sticky
         s.synchronize()
     assert e.value.code == -6
-    d = s.diagnostics()
-    assert d["nonfinite"] >= 1 and d["n"] == len(ic["mass"])
+    d = s.diagnostics()                               # still readable, with the status
+    assert d["nonfinite"] >= 1 and d["n"] == len(ic["mass"]) and d["status"] == -6
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+def test_reach_beyond_the_cell_layout_is_reported_by_step_and_run(precision, gpu):
+    """A smoothing length set from outside far above 2 h_init reaches past two
+    slice blocks of a column; the walk skips those pairs and flags it, and the
+    very step (or run) that met it returns DSPH_E_RANGE.  The kernels only set a
+    flag and finish normally."""
+    dx = 10.0
+    i, j, k = np.meshgrid(np.arange(3), np.arange(3), np.arange(150), indexing="ij")
+    pos = np.stack([i.ravel(), j.ravel(), -k.ravel()], 1).astype(float) * dx      # a 1.5 km tall column
+    n = len(pos)
+    mats = M.impact_materials(False)
+    prm = M.default_params(dx, pos.min(0) - 4 * dx, pos.max(0) + 4 * dx, False)
+    for use_run in (False, True):
+        s = dsph.System(prm, mats, precision)
+        s.add(pos, np.zeros((n, 3)), np.full(n, mats[0]["rho0"] * dx ** 3), np.zeros(n), np.zeros(n, np.int32))
+        s.init()
+        assert s.diagnostics()["status"] == 0
+        h = s.get("hh")
+        h[n // 2] = 30.0 * prm["h_init"]               # reach 2h = 780 m: some 320 slices of 4.9 m
+        s.set("hh", h)
+        with pytest.raises(dsph.DsphError) as e:
+            if use_run:
+                s.run(2, 1e-4, True)
+            else:
+                s.step(1e-4)
+        assert e.value.code == -5 and "two slice blocks" in str(e.value), str(e.value)
+        assert s.diagnostics()["status"] == -5
+        s.close()
 
 
 def test_autotune_keeps_state(gpu):
