@@ -1,21 +1,32 @@
 """Throughput of the MI355X SPH impact step (one JSON line on stdout).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload graze_1m] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload layered_4m]
+                    [--preroll P] [--run-steps T] [--no-cpu] [--no-fp64] [--no-ensemble]
 
 A step is one System::step of every particle of the workload (kick/drift, cell
 sort, adaptive-h density, EOS, strain + forces, damage, finish), with the CFL
-time step kept on the device.  The default workload is the ~1.05M-particle
-grazing impact (iron sphere R = 40 dx at 3 deg and 8 km/s into a 128x64x96
-granite slab, strength and Weibull damage on), mixed precision.  Inputs are
-resident in HBM before the timed region; K steps are replayed from a captured
-hipGraph and bracketed by device synchronisation.
+time step kept on the device.  The default workload is the ~4.0M-particle
+layered impact (iron sphere R = 60 dx at 2 deg and 8 km/s into 192x96 columns
+of 32 water cells over 136 granite cells, strength and Weibull damage on, 1000
+steps), mixed precision.  Inputs are resident in HBM before any timed region.
+
+Schedule of each precision's run: init, P steps that carry the impact well past
+contact (default: half the workload's step count), the autotuner on that
+post-contact state, W warm-up steps, then the K timed steps (replayed from a
+captured hipGraph and bracketed by device synchronisation), a few profiled
+steps, and the rest of the workload's steps.  `value` is the K-step window;
+`run` reports the whole workload (every step but the profiled ones timed).
 
 Besides the contract fields the line carries:
-  roofline     -- dominant kernel's algorithmic bytes / its mean HIP-event time
-                  (a profiled pass over the same workload right after the timed one);
+  roofline     -- dominant kernel's algorithmic HBM bytes / its mean HIP-event
+                  time over the profiled steps, its measured traffic and pipe
+                  utilisation from the committed rocprof summaries of these
+                  device sources, and its algorithmic FP32 rate;
   cpu_baseline -- the reference solver's own code (oracle/_ref, compiled from the
-                  reference headers) timed on this host on a bounded sample;
-  fp64         -- the all-FP64 device path on the same workload and its ratio.
+                  reference headers) timed on this host: init and at least 3 steps
+                  of the whole workload;
+  fp64         -- the all-FP64 device path on the same schedule and its ratio;
+  ensemble_256 -- runs/hr and step-level HBM fraction of the 256-impact batch.
 """
 from __future__ import annotations
 
@@ -52,22 +63,13 @@ def log(*a):
 
 
 def timed_run(sysm, steps, graph=True):
+    if steps <= 0:
+        return 0.0
     sysm.synchronize()
     t0 = time.perf_counter()
     sysm.run(steps, 0.0, graph)
     sysm.synchronize()
     return time.perf_counter() - t0
-
-
-def gpu_leg(ic, precision, warmup, steps, autotune=True):
-    s = dsph.make_batch(ic, precision) if isinstance(ic, list) else dsph.make_system(ic, precision)
-    s.init()
-    if autotune:
-        chosen = s.autotune(3)
-        log(f"autotune ({'fp64' if precision == dsph.FP64 else 'mixed'}): {chosen}")
-    s.run(warmup, 0.0, True)
-    el = timed_run(s, steps, True)
-    return s, el
 
 
 def profile_leg(s, steps):
@@ -78,6 +80,32 @@ def profile_leg(s, steps):
     prof = s.profile()
     s.set_profiling(False)
     return prof
+
+
+def gpu_leg(ic, precision, preroll, warmup, steps, run_steps, autotune=True, profile_steps=0):
+    """One precision's run on the schedule of the module docstring.  Returns the
+    system, the K-step time, the profile of the profiled steps and the whole-run
+    timing."""
+    name = "fp64" if precision == dsph.FP64 else "mixed"
+    s = dsph.make_batch(ic, precision) if isinstance(ic, list) else dsph.make_system(ic, precision)
+    s.init()
+    t_pre = timed_run(s, preroll)
+    d = s.diagnostics()
+    log(f"{name}: {preroll} steps before the window ({t_pre:.2f} s), t={d['t'] * 1e3:.2f} ms, "
+        f"umax={d['umax']:.3g} J/kg, dmax={d['dmax']:.3f}")
+    if autotune:
+        chosen = s.autotune(3)
+        log(f"autotune ({name}, at step {preroll}): {chosen}")
+    t_warm = timed_run(s, warmup)
+    el = timed_run(s, steps)
+    prof = profile_leg(s, profile_steps) if profile_steps else None
+    rest = max(0, run_steps - preroll - warmup - steps - profile_steps)
+    t_rest = timed_run(s, rest)
+    timed = preroll + warmup + steps + rest
+    run = {"steps": preroll + warmup + steps + profile_steps + rest, "timed_steps": timed,
+           "ms_per_step": 1e3 * (t_pre + t_warm + el + t_rest) / max(1, timed),
+           "t_sim_end": s.diagnostics()["t"]}
+    return s, el, prof, run
 
 
 def host_cores() -> dict:
@@ -107,8 +135,8 @@ def host_cores() -> dict:
     return {"cpu_model": model, "nproc": nproc, "usable_cores": usable, "cgroup_cpu_quota": quota}
 
 
-def time_cpu_solver(ic, which, nthreads, seconds_budget):
-    """Init, then as many steps as fit the budget (1 to 20); median step time."""
+def time_cpu_solver(ic, which, nthreads, seconds_budget, min_steps=3, max_steps=20):
+    """Init, then as many steps as fit the budget (at least min_steps); median step time."""
     from oracle import pyoracle as po
     o = po.make_system(ic, ic["params"], ic["materials"], which, nthreads=nthreads,
                        seed_damage=12345 if ic["params"]["damage_on"] else None, dx=ic["dx"])
@@ -121,12 +149,12 @@ def time_cpu_solver(ic, which, nthreads, seconds_budget):
         t0 = time.perf_counter()
         o.step(dt)
         times.append(time.perf_counter() - t0)
-        if sum(times) + t_init > seconds_budget or len(times) >= 20:
+        if len(times) >= max_steps or (len(times) >= min_steps and sum(times) + t_init > seconds_budget):
             break
     return float(np.median(times)), len(times), t_init
 
 
-def cpu_baseline(ic, seconds_budget=25.0):
+def cpu_baseline(ic, seconds_budget=30.0):
     """The reference solver on the host with its own thread cap (min(16, hardware
     threads), sph.hpp:58-60); when this process may use more cores than that, the
     oracle restatement (bit-identical to the reference) with the cap lifted too."""
@@ -138,8 +166,9 @@ def cpu_baseline(ic, seconds_budget=25.0):
     cores = min(16, host["nproc"])
     med, nsteps, t_init = time_cpu_solver(ic, which, cores, seconds_budget)
     out = {"value": len(ic["mass"]) / med, "unit": "particle-steps/s", "cores": cores, "kind": kind,
-           "sample": f"{nsteps} step(s) of the same workload after init, median {med * 1e3:.0f} ms/step "
-                     f"(init {t_init:.1f} s excluded); std::thread parallel_for over {cores} host threads",
+           "sample": f"init and {nsteps} step(s) of the whole workload ({len(ic['mass'])} particles), median "
+                     f"{med * 1e3:.0f} ms/step (init {t_init:.1f} s excluded); std::thread parallel_for over "
+                     f"{cores} host threads",
            "host": host}
     if host["usable_cores"] > cores:
         med2, n2, _ = time_cpu_solver(ic, "oracle", host["usable_cores"], seconds_budget)
@@ -152,15 +181,37 @@ def cpu_baseline(ic, seconds_budget=25.0):
     return out
 
 
+def ensemble_leg(warmup, steps, preroll, autotune=True):
+    """The 256-impact batch: runs/hr from the K-step window after `preroll` steps,
+    and the step-level algorithmic HBM fraction."""
+    t0 = time.perf_counter()
+    ic = ics.ensemble(256, ENSEMBLE_STEPS)
+    n = sum(len(m["mass"]) for m in ic)
+    log(f"ensemble of {len(ic)} impacts: n={n} (built in {time.perf_counter() - t0:.1f} s)")
+    s, el, _, _ = gpu_leg(ic, dsph.MIXED, preroll, warmup, steps, 0, autotune)
+    s.close()
+    value = n * steps / el
+    return {"runs_per_hour": len(ic) * 3600.0 / (ENSEMBLE_STEPS * el / steps), "particles": n,
+            "particle_steps_per_s": value, "ms_per_step": 1e3 * el / steps, "steps": steps,
+            "window": f"steps {preroll + warmup + 1}-{preroll + warmup + steps} of {ENSEMBLE_STEPS}",
+            "step_hbm_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="graze_1m", choices=list(WORKLOADS))
-    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--workload", default="layered_4m", choices=list(WORKLOADS))
+    ap.add_argument("--preroll", type=int, default=None,
+                    help="steps before the autotuner and the timed window (default: half the workload's steps)")
+    ap.add_argument("--run-steps", type=int, default=None,
+                    help="steps of the whole run (default: the workload's step count; 0 = stop after the window)")
+    ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-fast", action="store_true", help="skip the DSPH_MIXED_FAST leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the library's default tuning instead of the autotuner's choice")
     ap.add_argument("--copy-probe", action="store_true",
@@ -184,22 +235,29 @@ def main():
     if args.workload == "ensemble_256":
         ic = ics.ensemble(256, ENSEMBLE_STEPS)
         n = sum(len(m["mass"]) for m in ic)
+        cfg_steps = ENSEMBLE_STEPS
         log(f"workload ensemble of {len(ic)} impacts: n={n}")
         cpu_ic = sorted(ic, key=lambda m: len(m["mass"]))[len(ic) // 2]     # the median instance
     else:
         ic = WORKLOADS[args.workload]()
         n = len(ic["mass"])
-        log(f"workload {ic['meta']['name']}: n={n}")
-        cpu_ic = ics.impact_region(ic, 1_100_000)     # the CPU solver's share of a large impact
+        cfg_steps = int(ic["meta"]["steps"])
+        log(f"workload {ic['meta']['name']}: n={n}, {cfg_steps} steps")
+        cpu_ic = ic
+    preroll = cfg_steps // 2 if args.preroll is None else args.preroll
+    run_steps = cfg_steps if args.run_steps is None else args.run_steps
+    tune = not args.no_autotune
 
-    s, el = gpu_leg(ic, dsph.MIXED, args.warmup, args.steps, not args.no_autotune)
+    s, el, prof, run = gpu_leg(ic, dsph.MIXED, preroll, args.warmup, args.steps, run_steps, tune,
+                               args.profile_steps)
     value = n * args.steps / el
     ms = 1e3 * el / args.steps
-    log(f"mixed: {ms:.3f} ms/step  {value / 1e6:.1f} M particle-steps/s")
-    prof = profile_leg(s, args.profile_steps)
+    log(f"mixed: {ms:.3f} ms/step  {value / 1e6:.1f} M particle-steps/s (steps {preroll + args.warmup + 1}-"
+        f"{preroll + args.warmup + args.steps}); whole run {run['ms_per_step']:.3f} ms/step over "
+        f"{run['timed_steps']} steps")
     for k, (tot, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         if cnt:
-            log(f"  {k:11s} {tot / args.profile_steps:8.3f} ms/step  ({cnt} launches)")
+            log(f"  {k:14s} {tot / args.profile_steps:8.3f} ms/step  ({cnt} launches)")
     kb = roofline.kernel_bytes("mixed")
     # dominant kernel: the force pass, over neighbour lists or by cell walks
     # (whichever the tuning selected)
@@ -208,39 +266,63 @@ def main():
     kname = "k_force_list<float, false>" if phase == "force_list" else "k_force<float, false>"
     force_ms = prof[phase][0] / max(1, prof[phase][1])
     achieved = kb["force"] * n / (force_ms * 1e-3) / 1e9
-    # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile of this
-    # kernel and workload, used only when it was recorded for these device sources
-    rec = roofline.recorded_traffic(os.path.join(ROOT, "profiles"), kname, args.workload)
-    roof = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
+    force_tflops = roofline.force_flops() * n / (force_ms * 1e-3) / 1e12
+    # HBM bytes per launch and pipe utilisation from the committed rocprof summaries
+    # of this kernel and workload, used only when recorded for these device sources
+    pdir = os.path.join(ROOT, "profiles")
+    rec = roofline.recorded_traffic(pdir, kname, args.workload)
+    pipes = roofline.recorded_pipes(pdir, kname, args.workload)
+    roof = {"bound": roofline.limiter(pipes), "kernel": kname, "achieved": round(achieved, 2),
             "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5),
             "traffic": None if rec is None else rec["bytes"],
             "bytes_per_particle": kb["force"], "algorithmic_bytes": kb["force"] * n,
-            "mean_launch_ms": round(force_ms, 4), "source_digest": roofline.source_digest(),
+            "mean_launch_ms": round(force_ms, 4), "launches": prof[phase][1],
+            "source_digest": roofline.source_digest(), "pipes": pipes,
             "step_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5),
-            # the other roofline of the step: algorithmic FP32 flops against the vector peak
-            "valu": {"flops_per_particle_step": roofline.step_flops(),
-                     "achieved_tflops": round(roofline.step_flops() * value / 1e12, 3),
+            # the kernel against the vector ALU: algorithmic FP32 pair flops per launch
+            "valu": {"flops_per_particle": roofline.force_flops(), "achieved_tflops": round(force_tflops, 3),
                      "peak_tflops": roofline.MI355X_FP32_TFLOPS,
-                     "frac": round(roofline.step_flops() * value / 1e12 / roofline.MI355X_FP32_TFLOPS, 5)}}
+                     "frac": round(force_tflops / roofline.MI355X_FP32_TFLOPS, 5)},
+            "step_valu": {"flops_per_particle_step": roofline.step_flops(),
+                          "achieved_tflops": round(roofline.step_flops() * value / 1e12, 3),
+                          "frac": round(roofline.step_flops() * value / 1e12 / roofline.MI355X_FP32_TFLOPS, 5)}}
     diag = s.diagnostics()
     s.close()
 
     fp64 = None
     if not args.no_fp64:
-        s64, el64 = gpu_leg(ic, dsph.FP64, min(args.warmup, 5), max(5, args.steps // 4), not args.no_autotune)
-        v64 = n * max(5, args.steps // 4) / el64
+        s64, el64, _, run64 = gpu_leg(ic, dsph.FP64, preroll, args.warmup, args.steps, run_steps, tune)
+        v64 = n * args.steps / el64
         s64.close()
-        fp64 = {"value": v64, "ms_per_step": 1e3 * el64 / max(5, args.steps // 4),
-                "mixed_over_fp64": value / v64}
-        log(f"fp64: {fp64['ms_per_step']:.3f} ms/step  ratio {fp64['mixed_over_fp64']:.2f}x")
+        fp64 = {"value": v64, "ms_per_step": 1e3 * el64 / args.steps, "mixed_over_fp64": value / v64,
+                "run_ms_per_step": run64["ms_per_step"],
+                "run_mixed_over_fp64": run64["ms_per_step"] / run["ms_per_step"] if run["timed_steps"] else None}
+        log(f"fp64: {fp64['ms_per_step']:.3f} ms/step  ratio {fp64['mixed_over_fp64']:.2f}x; whole run "
+            f"{run64['ms_per_step']:.3f} ms/step")
+
+    fast = None
+    if not args.no_fast:
+        sf, elf, _, runf = gpu_leg(ic, dsph.MIXED_FAST, preroll, args.warmup, args.steps, run_steps, tune)
+        sf.close()
+        vf = n * args.steps / elf
+        fast = {"value": vf, "ms_per_step": 1e3 * elf / args.steps, "run_ms_per_step": runf["ms_per_step"],
+                "precision": "FP32 density sums and pressure as well (per-particle P ~5e-3 near rho0)"}
+        if fp64 is not None:
+            fast["over_fp64"] = vf / fp64["value"]
+        log(f"mixed_fast: {fast['ms_per_step']:.3f} ms/step; whole run {runf['ms_per_step']:.3f} ms/step")
+
+    ens = None
+    if not args.no_ensemble and args.workload != "ensemble_256":
+        ens = ensemble_leg(args.warmup, args.steps, ENSEMBLE_STEPS // 2, tune)
+        log(f"ensemble_256: {ens['ms_per_step']:.3f} ms/step, {ens['runs_per_hour']:.0f} runs/hr, "
+            f"step HBM fraction {ens['step_hbm_frac']:.4f}")
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cpu_ic)
         if cpu_ic is not ic:
-            what = "median-size ensemble instance" if args.workload == "ensemble_256" else "impact region"
-            cpu["sample"] = f"{what} ({len(cpu_ic['mass'])} particles): " + cpu["sample"]
+            cpu["sample"] = f"median-size ensemble instance ({len(cpu_ic['mass'])} particles): " + cpu["sample"]
         log(f"cpu: {cpu['value'] / 1e6:.3f} M particle-steps/s ({cpu['kind']}, {cpu['cores']} threads); "
             f"all {cpu['all_cores']['cores']} usable cores: {cpu['all_cores']['value'] / 1e6:.3f} M/s; "
             f"{cpu['host']['cpu_model']}, nproc {cpu['host']['nproc']}")
@@ -249,12 +331,18 @@ def main():
         "metric": "SPH particle-steps/s on 1 MI355X, mixed vs FP64 at stated error; runs/hr",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 pair/EOS/stress, f64 state+reductions",
+        "vs_baseline": None,
+        "dtype": "f32 force pass, stress, sound speed; f64 density terms, pressure, state, reductions",
         "data": "synthetic (seeded lattice impact IC)",
         "config": {"workload": WORKLOAD[args.workload], "particles": n, "dt": "device CFL (cfl=0.2)",
-                   "graph": True, "t_sim_end": diag["t"]},
-        "roofline": roof, "cpu_baseline": cpu, "fp64": fp64,
+                   "graph": True, "window": f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps}"
+                   f" of {cfg_steps} (autotuned at step {preroll})" if tune else
+                   f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps} of {cfg_steps}",
+                   "t_sim_end": diag["t"]},
+        "run": run, "roofline": roof, "cpu_baseline": cpu, "fp64": fp64, "mixed_fast": fast,
     }
+    if ens is not None:
+        line["ensemble_256"] = ens
     if args.workload == "ensemble_256":
         # every batched step advances all 256 instances by one step
         line["runs_per_hour"] = len(ic) * 3600.0 / (ENSEMBLE_STEPS * el / args.steps)

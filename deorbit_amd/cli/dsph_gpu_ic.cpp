@@ -1,7 +1,7 @@
 // Drop-in for the reference's production GPU driver (hydrocodes/sph/gpu_ic.cpp):
 //
 //   dsph_gpu_ic <ic.bin> <dx> <t_end> [walltime] [--frac] [--g g] [--snap dt] [--damp f]
-//               [--fp64] [--materials basalt,iron,basalt] [--graph-steps k]
+//               [--fp64 | --fast] [--materials basalt,iron,basalt] [--graph-steps k]
 //               [--checkpoint file] [--checkpoint-every dt] [--restart file] [--max-steps n]
 //
 // Same arguments, IC format (gpu_ic.cpp:44-51), snapshot files gpu_snap_NNN.bin
@@ -163,7 +163,7 @@ struct Snapshotter {
 }  // namespace
 
 int main(int argc, char** argv) {
-    bool frac = false, fp64 = false;
+    bool frac = false, fp64 = false, fast = false;
     double gz = 0.0, snap_dt = 0.05, damp = 1.0;
     int graph_steps = 25;
     std::string mats = "basalt,iron,basalt";       // gpu_ic.cpp:67 (material 2 = trackable basalt impactor)
@@ -175,6 +175,7 @@ int main(int argc, char** argv) {
         std::string s = argv[i];
         if (s == "--frac") frac = true;
         else if (s == "--fp64") fp64 = true;
+        else if (s == "--fast") fast = true;     // DSPH_MIXED_FAST: FP32 density sums too
         else if (s == "--g" && i + 1 < argc) gz = std::atof(argv[++i]);
         else if (s == "--snap" && i + 1 < argc) snap_dt = std::atof(argv[++i]);
         else if (s == "--damp" && i + 1 < argc) damp = std::atof(argv[++i]);
@@ -188,7 +189,7 @@ int main(int argc, char** argv) {
     }
     if (av.size() < 4) {
         std::printf("usage: dsph_gpu_ic ic.bin dx t_end [walltime] [--frac] [--g g_mars] [--snap dt] [--damp f]"
-                    " [--fp64] [--materials a,b,c] [--checkpoint file] [--checkpoint-every dt] [--restart file]"
+                    " [--fp64 | --fast] [--materials a,b,c] [--checkpoint file] [--checkpoint-every dt] [--restart file]"
                     " [--max-steps n]\n");
         return 1;
     }
@@ -201,7 +202,7 @@ int main(int argc, char** argv) {
     S.eta = 1.3; S.h_init = 1.3 * dx; S.alpha = 1.0; S.beta = 2.0; S.eps = 0.01; S.cfl = 0.2;
     S.eps_as = frac ? 0.3 : 0.0; S.damage_on = frac;
     S.g = gz; S.damp = damp; S.t_end = t_end;
-    S.precision = fp64 ? DSPH_FP64 : DSPH_MIXED;
+    S.precision = fp64 ? DSPH_FP64 : (fast ? DSPH_MIXED_FAST : DSPH_MIXED);
     auto add_row = [&](long i, const double* r) -> bool {
         S.cur_mat = (int)r[8];
         if (S.cur_mat < 0 || S.cur_mat >= (int)S.materials.size()) {
@@ -248,7 +249,8 @@ int main(int argc, char** argv) {
         }
     }
     std::printf("loaded %s: n=%ld (%d projectile)  dx=%.0f  frac=%d  g=%.2f  precision=%s\n",
-                restart ? restart_path.c_str() : icf, n, n_proj, dx, frac, gz, fp64 ? "fp64" : "mixed");
+                restart ? restart_path.c_str() : icf, n, n_proj, dx, frac, gz,
+                fp64 ? "fp64" : (fast ? "mixed_fast" : "mixed"));
     try {
         if (restart) {
             std::vector<double> D(n), ea(n), Sm(6 * (size_t)n);

@@ -104,6 +104,7 @@ struct Work {
     R *dudt, *dSdt;                  // from the last force pass (dSdt: 6 x n)
     R *dudt_old, *dSdt_old;          // previous force pass, in the current order
     V4* dens_rec;                    // (position key, mass)           density pass
+    double4* dens_rec64;             // mixed path: (x, y, z, m) in FP64, for the FP64 density terms
     V4* frec_pos;                    // (position key, h)              force pass
     V4* frec_vel;                    // (velocity, mass)
     V4* frec_th;                     // (rho, Pf/rho^2, csig, Rart)
@@ -129,6 +130,7 @@ struct Work {
     R skin_check;                    // lists stay valid while h <= skin_check * h0
     BlockPart* parts;                // per-block partial reductions
     const MatT<R>* mats;
+    const MatT<double>* mats64;      // the material table in FP64 (pressure from an FP64 density)
     const SegPar* par;
     SegGrid* grid;
     SegState* st;

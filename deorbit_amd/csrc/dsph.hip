@@ -73,8 +73,11 @@ class Engine : public EngineBase {
   public:
     using V4 = typename Vec4<R>::T;
 
-    Engine(const std::vector<dsph_params>& segp, const std::vector<dsph_material>& mats, hipStream_t s)
-        : segp_(segp), mats_(mats), stream_(s) {
+    // density64: the mixed path's density with FP64 pair terms and the pressure
+    // in FP64 (DSPH_MIXED); false keeps FP32 density sums (DSPH_MIXED_FAST)
+    Engine(const std::vector<dsph_params>& segp, const std::vector<dsph_material>& mats, hipStream_t s,
+           bool density64)
+        : segp_(segp), mats_(mats), stream_(s), density64_(sizeof(R) == 4 && density64) {
         nseg_ = (int)segp.size();
         tune_.block_pair = 256;
         tune_.block_point = 256;
@@ -143,6 +146,7 @@ class Engine : public EngineBase {
         w.dudt = dalloc<R>(n_); w.dSdt = dalloc<R>(6 * n_);
         w.dudt_old = dalloc<R>(n_); w.dSdt_old = dalloc<R>(6 * n_);
         w.dens_rec = dalloc<V4>(n_);
+        w.dens_rec64 = density64_ ? dalloc<double4>(n_) : nullptr;
         w.frec_pos = dalloc<V4>(n_); w.frec_vel = dalloc<V4>(n_); w.frec_th = dalloc<V4>(n_);
         w.frec_sa = dalloc<V4>(n_); w.frec_sb = dalloc<V4>(n_);
         w.cellc = dalloc<uint32_t>(n_);
@@ -164,6 +168,11 @@ class Engine : public EngineBase {
         for (size_t k = 0; k < mats_.size(); k++) hm[k] = to_mat(mats_[k]);
         d_mats_ = dalloc<MatT<R>>(hm.size());
         HIPC(hipMemcpyAsync((void*)d_mats_, hm.data(), hm.size() * sizeof(MatT<R>), hipMemcpyHostToDevice, stream_));
+        std::vector<MatT<double>> hm64(mats_.size());
+        for (size_t k = 0; k < mats_.size(); k++) hm64[k] = Engine<double>::to_mat(mats_[k]);
+        d_mats64_ = dalloc<MatT<double>>(hm64.size());
+        HIPC(hipMemcpyAsync((void*)d_mats64_, hm64.data(), hm64.size() * sizeof(MatT<double>), hipMemcpyHostToDevice,
+                            stream_));
         std::vector<SegPar> hpar(nseg_);
         for (int s = 0; s < nseg_; s++) hpar[s] = to_par(segp_[s]);
         d_par_ = dalloc<SegPar>(nseg_);
@@ -179,6 +188,7 @@ class Engine : public EngineBase {
         HIPC(hipMemcpyAsync(d_st_, hst.data(), nseg_ * sizeof(SegState), hipMemcpyHostToDevice, stream_));
         d_segdt_ = dalloc<double>(nseg_);
         w.mats = d_mats_;
+        w.mats64 = d_mats64_;
         w.par = d_par_;
         w.grid = d_grid_;
         w.st = d_st_;
@@ -231,7 +241,7 @@ class Engine : public EngineBase {
     // autotuner also tries lists of 128 entries (about 1.5x the mean list
     // length of a uniform lattice at the default skin).
     int list_capacity() const {
-        if (tune_.neighbor_capacity < 0) return 0;
+        if (density64_ || tune_.neighbor_capacity < 0) return 0;
         if (tune_.neighbor_capacity > 0) return tune_.neighbor_capacity;
         return 0;
     }
@@ -461,6 +471,14 @@ class Engine : public EngineBase {
         for (int r = 0; r < 4; r++) {
             launch(PH_ROUND, "round", k_round, dim3(1), dim3(256), d_st_, d_grid_, (const SegPar*)d_par_, nseg_, r);
             const auto th = hmax_target(r + 1);
+            if constexpr (sizeof(R) == 4) {
+                if (density64_) {
+                    launch(PH_DENSITY, "density", k_density_round_hp<PER>, dim3(nb), dim3(bq), cur(),
+                           w_, r, tune_.xcd_remap, th);
+                    fold_parts(PH_DENSITY, th, nb);
+                    continue;
+                }
+            }
             if (r == 0 && lists) {
                 launch(PH_DENSITY_BUILD, "density_build", k_density_build<R, PER>, dim3(nb), dim3(bq), cur(), w_,
                        tune_.xcd_remap);
@@ -537,7 +555,10 @@ class Engine : public EngineBase {
         }
         if (!records_) {
             const int bp = tune_.block_point;
-            launch(PH_EOS, "eos_records", k_eos_records<R>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_);
+            if (density64_)
+                launch(PH_EOS, "eos_records", k_eos_records<R, true>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_);
+            else
+                launch(PH_EOS, "eos_records", k_eos_records<R, false>, dim3(blocks_for(n_, bp)), dim3(bp), cur(), w_);
             records_ = true;
         }
         force();
@@ -863,6 +884,8 @@ class Engine : public EngineBase {
         if (t.neighbor_capacity) {
             if (t.neighbor_capacity > 0 && (t.neighbor_capacity < 8 || t.neighbor_capacity > 4096))
                 throw Err{DSPH_E_ARG, "neighbor_capacity must be in [8, 4096], or negative for no lists"};
+            if (t.neighbor_capacity > 0 && density64_)
+                throw Err{DSPH_E_ARG, "neighbour lists serve the FP32 density sums: use DSPH_MIXED_FAST or DSPH_FP64"};
             nt.neighbor_capacity = t.neighbor_capacity > 0 ? t.neighbor_capacity : -1;
         }
         const bool lists_changed = nt.neighbor_capacity != tune_.neighbor_capacity || nt.list_skin != tune_.list_skin;
@@ -954,6 +977,7 @@ class Engine : public EngineBase {
         };
         for (int cap : caps)
         for (double cf : cells) {
+            if (cap > 0 && density64_) continue;      // the FP64 density walks the cells
             for (int bq : blocks) {
                 release_graphs();
                 set_lists(cap);
@@ -1037,6 +1061,8 @@ class Engine : public EngineBase {
     int nb_sort_ = 0, passes_ = 0, nparts_max_ = 0;
     uint32_t capacity_ = 0;
     MatT<R>* d_mats_ = nullptr;
+    MatT<double>* d_mats64_ = nullptr;
+    bool density64_ = false;
     SegPar* d_par_ = nullptr;
     SegGrid* d_grid_ = nullptr;
     SegState* d_st_ = nullptr;
@@ -1132,7 +1158,8 @@ int create_impl(const dsph_params* params, int32_t nseg, const dsph_material* ma
     require(out != nullptr && params != nullptr && mats != nullptr, DSPH_E_ARG, "null argument");
     require(nseg >= 1 && nseg <= 65535, DSPH_E_ARG, "segment count must be in [1, 65535]");
     require(nmat >= 1 && nmat <= 256, DSPH_E_ARG, "material count must be in [1, 256]");
-    require(precision == DSPH_MIXED || precision == DSPH_FP64, DSPH_E_ARG, "precision must be DSPH_MIXED or DSPH_FP64");
+    require(precision == DSPH_MIXED || precision == DSPH_FP64 || precision == DSPH_MIXED_FAST, DSPH_E_ARG,
+            "precision must be DSPH_MIXED, DSPH_FP64 or DSPH_MIXED_FAST");
     for (int s = 0; s < nseg; s++) validate_params(params[s]);
     for (int m = 0; m < nmat; m++) {
         require(mats[m].type == DSPH_EOS_IDEAL || mats[m].type == DSPH_EOS_TILLOTSON, DSPH_E_ARG, "bad EOS type");
@@ -1149,8 +1176,8 @@ int create_impl(const dsph_params* params, int32_t nseg, const dsph_material* ma
     c->precision = precision;
     c->segp.assign(params, params + nseg);
     c->mats.assign(mats, mats + nmat);
-    if (precision == DSPH_FP64) c->ed = std::make_unique<Engine<double>>(c->segp, c->mats, c->stream);
-    else c->ef = std::make_unique<Engine<float>>(c->segp, c->mats, c->stream);
+    if (precision == DSPH_FP64) c->ed = std::make_unique<Engine<double>>(c->segp, c->mats, c->stream, false);
+    else c->ef = std::make_unique<Engine<float>>(c->segp, c->mats, c->stream, precision == DSPH_MIXED);
     *out = c.release();
     return DSPH_OK;
 }
@@ -1159,7 +1186,7 @@ int create_impl(const dsph_params* params, int32_t nseg, const dsph_material* ma
 
 extern "C" {
 
-const char* dsph_version(void) { return "dsph 0.1 (gfx950, abi 1)"; }
+const char* dsph_version(void) { return "dsph 0.2 (gfx950, abi 2)"; }
 const char* dsph_last_error(void) { return g_last_error.c_str(); }
 
 int dsph_create(const dsph_params* params, const dsph_material* mats, int32_t nmat, int32_t precision,

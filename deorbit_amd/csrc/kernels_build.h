@@ -458,13 +458,16 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
             w.cellc[i] = cellc_pack(cx, cy, cz);
             V4 rec;
             if (sizeof(R) == 4) {
-                rec.x = (R)(x - (g.lo[0] + (double)cx * g.cw[0]));
-                rec.y = (R)(y - (g.lo[1] + (double)cy * g.cw[1]));
-                rec.z = (R)(z - (g.lo[2] + (double)block_origin(cz) * g.cw[2]));
+                const double ox = x - (g.lo[0] + (double)cx * g.cw[0]);
+                const double oy = y - (g.lo[1] + (double)cy * g.cw[1]);
+                const double oz = z - (g.lo[2] + (double)block_origin(cz) * g.cw[2]);
+                rec.x = (R)ox; rec.y = (R)oy; rec.z = (R)oz;
+                rec.w = (R)m;
+                if (w.dens_rec64) w.dens_rec64[i] = make_double4(x, y, z, m);   // the FP64 density terms
             } else {
                 rec.x = (R)x; rec.y = (R)y; rec.z = (R)z;
+                rec.w = (R)m;
             }
-            rec.w = (R)m;
             w.dens_rec[i] = rec;
         } else {
             w.cellc[i] = 0u;
@@ -472,6 +475,7 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
             rec.x = rec.y = rec.z = (R)__builtin_nan("");
             rec.w = (R)0;
             w.dens_rec[i] = rec;
+            if (sizeof(R) == 4 && w.dens_rec64) w.dens_rec64[i] = make_double4(x, y, z, 0.0);
         }
         hv = (double)h;
     }

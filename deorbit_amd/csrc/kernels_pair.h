@@ -80,8 +80,8 @@ struct AxisWalk {
             cull = false;
         }
     }
-    // false when offset d leaves a non-periodic axis
-    __device__ __forceinline__ bool cell(int d, int& n, R& shift, R& gap) const {
+    // false when offset d leaves a non-periodic axis; ishift is the shift in cells
+    __device__ __forceinline__ bool cell(int d, int& n, R& shift, int& ishift, R& gap) const {
         n = c + d;
         if (per) {
             if (n < 0) n += dim;
@@ -91,7 +91,8 @@ struct AxisWalk {
         }
         // image frame corner: frame start of n, moved by the wrap (c + d - n)
         const int fs = n - n % blk, fc = c - c % blk;
-        shift = (R)(fs + (c + d - n) - fc) * cw;
+        ishift = fs + (c + d - n) - fc;
+        shift = (R)ishift * cw;
         gap = cull ? cell_gap(d, o, cw) : R(0);
         return true;
     }
@@ -152,14 +153,17 @@ __device__ __forceinline__ bool oracle_reaches(const SegGrid& g, const SegPar& p
 
 // Per-particle EOS, damage scaling and artificial-stress coefficient
 // (sph.hpp:261-275), then the force-pass neighbour records of particle i.
-template <class R>
+// With P64 the pressure is evaluated in FP64 from rho64, the FP64 density
+// (mixed path): P ~ A (rho/rho0 - 1) magnifies a relative density error by
+// 1/|rho/rho0 - 1|, which is ~1e4 for iron at 10 MPa.
+template <class R, bool P64 = false>
 __device__ __forceinline__ void eos_epilogue(const Carried<R>& c, const Work<R>& w, int64_t i, const SegPar& p,
-                                             int mat, R rho, R h, R ox, R oy, R oz) {
+                                             int mat, R rho, R h, R ox, R oy, R oz, double rho64 = 0.0) {
     using V4 = typename Vec4<R>::T;
     const int64_t n = w.n;
     const MatT<R> m = w.mats[mat];
     const R u = (R)c.u[i];
-    const R P = eos_pressure(m, rho, u);
+    const R P = P64 ? (R)eos_pressure<double>(w.mats64[mat], rho64, c.u[i]) : eos_pressure(m, rho, u);
     const R cs = eos_sound_speed(m, rho, u);
     const R csig = d_sqrt(cs * cs + (R(4) / R(3)) * m.G / smax(rho, rho_floor<R>()));
     R s6[6];
@@ -305,6 +309,13 @@ struct DensLane {
     }
 };
 
+// Frame shift of a visited piece in whole cells (x, y) and slices (z, for the
+// slots below and from the piece's split), for walkers that form the shift in a
+// precision of their own: shift = multiple * cell width.
+struct PieceFrame {
+    int ix, iy, iz0, iz1;
+};
+
 // Floor to int in the working precision.
 __device__ __forceinline__ int ifloor(float x) { return (int)floorf(x); }
 __device__ __forceinline__ int ifloor(double x) { return (int)floor(x); }
@@ -315,7 +326,8 @@ constexpr float kSliceMargin = 1e-3f;
 // Visit the candidates of particle i as pieces: runs [k0, k1) of the sorted
 // order with one frame shift (sx, sy) across x and y and, along z, sz0 for
 // slots below ksplit and sz1 from ksplit on (see AxisWalk);
-// fn(k0, k1, ksplit, sx, sy, sz0, sz1, reach) per piece.  reach_max bounds
+// fn(k0, k1, ksplit, sx, sy, sz0, sz1, reach, frame) per piece (frame: the
+// same shifts as whole multiples of the cell widths).  reach_max bounds
 // the reach; with HB the reach of a piece is reach_of(its bound of h) and the
 // piece is culled with it (else reach is reach_max).  (ox, oy) are offsets
 // from the own column's corner, ozs from the own slice's corner.
@@ -383,7 +395,8 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                 if (k0 >= k1) continue;
                 const int g0 = block_start(z0), g1 = block_start(z1);
                 const uint32_t ks = (g1 != g0) ? w.cstart[col + g1] : k1;
-                fn(k0, k1, ks, (R)dx * wx, (R)dy * wy, (R)(g0 - fi) * wz, (R)(g1 - fi) * wz, rb);
+                fn(k0, k1, ks, (R)dx * wx, (R)dy * wy, (R)(g0 - fi) * wz, (R)(g1 - fi) * wz, rb,
+                   PieceFrame{dx, dy, g0 - fi, g1 - fi});
             }
         }
         return;
@@ -393,21 +406,21 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
     Y.init(g, 1, cy, oy, (double)reach_max);
     Z.init(g, 2, cz, ozs, (double)reach_max);
     for (int dx = X.lo; dx <= X.hi; dx++) {
-        int nx;
+        int nx, mx;
         R sx, gx;
-        if (!X.cell(dx, nx, sx, gx)) continue;
+        if (!X.cell(dx, nx, sx, mx, gx)) continue;
         const R gx2 = gx * gx;
         if (gx2 >= rmax2) continue;
         for (int dy = Y.lo; dy <= Y.hi; dy++) {
-            int ny;
+            int ny, my;
             R sy, gy;
-            if (!Y.cell(dy, ny, sy, gy)) continue;
+            if (!Y.cell(dy, ny, sy, my, gy)) continue;
             const R gxy = gx2 + gy * gy;
             if (gxy >= rmax2) continue;
             for (int dz = Z.lo; dz <= Z.hi; dz++) {
-                int nz;
+                int nz, mz;
                 R sz, gz;
-                if (!Z.cell(dz, nz, sz, gz)) continue;
+                if (!Z.cell(dz, nz, sz, mz, gz)) continue;
                 const R gg = gxy + gz * gz;
                 if (gg >= rmax2) continue;
                 R rb = reach_max;
@@ -417,7 +430,7 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                 }
                 const uint32_t key = cell_key(g, nx, ny, nz);
                 const uint32_t k0 = w.cstart[key], k1 = w.cstart[key + 1];
-                if (k0 < k1) fn(k0, k1, k1, sx, sy, sz, sz, rb);
+                if (k0 < k1) fn(k0, k1, k1, sx, sy, sz, sz, rb, PieceFrame{mx, my, mz, mz});
             }
         }
     }
@@ -431,7 +444,8 @@ __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p
     using V4 = typename Vec4<R>::T;
     R s = R(0);
     for_pieces<R, PER, false>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h, [](R) { return R(0); },
-                              [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1, R) {
+                              [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1, R,
+                                  const PieceFrame&) {
         R bx, by, bz0, bz1;
         L.frame(sx, sy, sz0, bx, by, bz0);
         L.frame(sx, sy, sz1, bx, by, bz1);
@@ -455,6 +469,100 @@ __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p
         }
     });
     return s;
+}
+
+// ---------------------------------------------------------------- FP64 density terms (mixed path)
+// sqrt(r2) from the FP32 reciprocal square root and one Newton step in FP64:
+// relative error ~1.5 e^2 for a seed error e ~1e-7 (r2 = 0 gives 0).
+__device__ __forceinline__ double sqrt_seeded(double r2) {
+    const float y = __builtin_amdgcn_rsqf(fmaxf((float)r2, 1e-30f));
+    const double yd = (double)y;
+    const double r0 = r2 * yd;
+    return fma(fma(-r0, r0, r2), 0.5 * yd, r0);
+}
+
+// FP64 quantities of particle i for the FP64 density walk: h and the absolute
+// position.
+struct DensHP {
+    double h, two_h, inv_h, x, y, z;
+    __device__ __forceinline__ void setup(double hh) {
+        h = hh;
+        two_h = 2.0 * hh;
+        inv_h = 1.0 / hh;
+    }
+};
+
+// sum_j m_j f(r_ij / h_i) over r_ij < 2 h_i with FP64 separations x_i - x_j
+// (as the reference forms them), kernel shapes, masses and sum; rho_i = sum /
+// (pi h_i^3).  Same candidates and order as density_walk: the walk and an FP32
+// screen of its records (range widened by 2e-5 in r^2) pick the candidates, the
+// FP64 test r < 2h (sph.hpp:191) decides.  Terms of the candidates screened out
+// are exact zeros, so they are skipped.
+template <bool PER>
+__device__ double density_walk_hp(const Carried<float>& c, const Work<float>& w, const SegPar& p,
+                                  const SegGrid& g, const DensLane<float, PER>& L, const DensHP& H) {
+    double s = 0.0;
+    auto term = [&](const double4& q) -> double {
+        double rx = H.x - q.x, ry = H.y - q.y, rz = H.z - q.z;
+        if (PER && L.per) min_image(g, rx, ry, rz);
+        const double r = sqrt_seeded(rx * rx + ry * ry + rz * rz);
+        const double qq = r * H.inv_h;
+        const double t = fmax(2.0 - qq, 0.0), sv = fmax(1.0 - qq, 0.0);
+        const double fv = 0.25 * t * t * t - sv * sv * sv;
+        return r < H.two_h ? q.w * fv : 0.0;
+    };
+    for_pieces<float, PER, false>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h * 1.00001f,
+                                  [](float) { return 0.0f; },
+                                  [&](uint32_t k0, uint32_t k1, uint32_t ks, float sx, float sy, float sz0, float sz1,
+                                      float, const PieceFrame&) {
+        float bx, by, bz0, bz1;
+        L.frame(sx, sy, sz0, bx, by, bz0);
+        L.frame(sx, sy, sz1, bx, by, bz1);
+        if (!L.emu) {
+            // in chunks of up to 64 candidates: the screen (four records in flight)
+            // sets a bit per candidate that may be in range, then the FP64 terms run
+            // over the set bits two at a time, in increasing k
+            const float th2 = L.th2 * 1.00002f;
+            for (uint32_t c0 = k0; c0 < k1; c0 += 64u) {
+                const uint32_t c1 = min(c0 + 64u, k1), last = c1 - 1;
+                uint64_t mask = 0;
+                for (uint32_t k = c0; k < c1; k += 4) {
+                    float4 q[4];
+#pragma unroll
+                    for (int b = 0; b < 4; b++) q[b] = w.dens_rec[min(k + b, last)];
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        float rx = bx - q[b].x, ry = by - q[b].y, rz = (k + b < ks ? bz0 : bz1) - q[b].z;
+                        if (PER && L.per) min_image(g, rx, ry, rz);
+                        const bool in = k + b < c1 && rx * rx + ry * ry + rz * rz < th2;
+                        mask |= (uint64_t)in << (k + b - c0);
+                    }
+                }
+                while (mask) {
+                    const uint32_t ka = c0 + (uint32_t)__builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    const bool two = mask != 0;
+                    const uint32_t kb = two ? c0 + (uint32_t)__builtin_ctzll(mask) : ka;
+                    if (two) mask &= mask - 1;
+                    const double4 qa = w.dens_rec64[ka], qb = w.dens_rec64[kb];
+                    s += term(qa);
+                    if (two) s += term(qb);
+                }
+            }
+        } else {
+            for (uint32_t k = k0; k < k1; k++) {
+                if (!oracle_reaches(g, p, L.ob, c.x[k], c.y[k], c.z[k])) continue;
+                s += term(w.dens_rec64[k]);
+            }
+        }
+    });
+    return s;
+}
+
+// The new h from an FP64 density (sph.hpp:195-198), in FP64.
+__device__ __forceinline__ double density_h64(const SegPar& p, double m, double rho) {
+    const double h0 = p.h_init;
+    return smin(2.0 * h0, smax(0.5 * h0, p.eta * cbrt(m / rho)));
 }
 
 // The new h of particle i from its density (sph.hpp:195-198).
@@ -588,6 +696,62 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
 }
 
+// Both iterations of density round r for the mixed path with FP64 density
+// terms (k_density_round with density_walk_hp): h of the first iteration stays
+// in FP64 for the second, the density and the new h are FP64 until stored, and
+// the pressure of the EOS epilogue is evaluated in FP64 from that density.
+// Both iterations need the FP64 terms: an FP32 first iteration moves h by
+// ~3e-7, which the second iteration's density passes on to P at up to ~1e-4
+// of |P| near 10 MPa.  Neighbour lists are not used.
+template <bool PER>
+__global__ __launch_bounds__(256) void k_density_round_hp(Carried<float> c, Work<float> w, int r, int remap,
+                                                         SegTargets<1> tg) {
+    const int b = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    int seg = 0;
+    bool act = false;
+    if (i < w.n) {
+        seg = info_seg(c.info[i]);
+        act = w.st[seg].active[r] != 0;
+    }
+    double hnew = 0.0;
+    uint32_t cc = 0;
+    bool fin = false;
+    if (act) {
+        const SegPar& p = w.par[seg];
+        const SegGrid& g = w.grid[seg];
+        DensLane<float, PER> L;
+        DensHP H;
+        cc = w.cellc[i];
+        L.me = w.dens_rec[i];
+        own_offset<float>(c, g, i, cc, L.me, L.ox, L.oy, L.oz);
+        const bool finite = isfinite(L.ox) && isfinite(L.oy) && isfinite(L.oz);
+        fin = finite;
+        H.x = c.x[i];
+        H.y = c.y[i];
+        H.z = c.z[i];
+        const double m = c.mass[i];
+        double h = (double)c.h[i], rho = 1e-30;
+        for (int it = 0; it < 2; it++) {
+            rho = 1e-30;
+            if (finite) {
+                L.h = (float)h;
+                L.setup(c, p, g, i, cc);
+                H.setup(h);
+                rho = smax(density_walk_hp<PER>(c, w, p, g, L, H) * (H.inv_h * H.inv_h * H.inv_h / kPi), 1e-30);
+            }
+            h = density_h64(p, m, rho);
+        }
+        hnew = h;
+        w.rho[i] = (float)rho;
+        c.h[i] = (float)h;
+        eos_epilogue<float, true>(c, w, i, p, info_mat(c.info[i]), (float)rho, (float)h, L.ox, L.oy, L.oz, rho);
+    }
+    raise_h_bound<float>(w, w.grid[seg], cc, hnew, act && fin);
+    const double v[1] = {hnew};
+    block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
+}
+
 // Second density iteration of round 0 for the particles whose list is usable:
 // the list holds the walk's pairs in the walk's order, so the sum is the one
 // the walk would give.  Four entries are read ahead so their loads overlap.
@@ -677,7 +841,7 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
             for_pieces<R, PER, true>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, sk * (L.h + smax(L.h, hm)),
                                      [&](R hb) { return sk * (L.h + smax(L.h, hb)); },
                                      [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1,
-                                         R rb) {
+                                         R rb, const PieceFrame&) {
                 const R rb2 = rb * rb;
                 R bx, by, bz0, bz1;
                 L.frame(sx, sy, sz0, bx, by, bz0);
@@ -723,7 +887,7 @@ __global__ __launch_bounds__(256) void k_density_build(Carried<R> c, Work<R> w, 
 }
 
 // EOS epilogue on its own (System::compute_forces entry without a density pass).
-template <class R>
+template <class R, bool P64>
 __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < w.n;
@@ -738,7 +902,7 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
         cc = w.cellc[i];
         R ox, oy, oz;
         own_offset<R>(c, g, i, cc, w.dens_rec[i], ox, oy, oz);
-        eos_epilogue<R>(c, w, i, w.par[seg], info_mat(inf), w.rho[i], c.h[i], ox, oy, oz);
+        eos_epilogue<R, P64>(c, w, i, w.par[seg], info_mat(inf), w.rho[i], c.h[i], ox, oy, oz, (double)w.rho[i]);
         h = (double)c.h[i];
         fin = isfinite((double)ox);
     }
@@ -944,29 +1108,42 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                 for_pieces<R, PER, true>(g, w, F.cx, F.cy, F.cz, F.ox, F.oy, F.ozs,
                                          F.hi + (R)w.st[seg].hmax_final, [&](R hb) { return F.hi + hb; },
                                          [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1,
-                                             R) {
+                                             R, const PieceFrame&) {
                     R bx, by, bz0, bz1;
                     if (sizeof(R) == 4) {
                         bx = F.ox - sx; by = F.oy - sy; bz0 = F.oz - sz0; bz1 = F.oz - sz1;
                     } else {
                         bx = F.pi.x; by = F.pi.y; bz0 = bz1 = F.pi.z;
                     }
-                    // candidate k: the other records are read only for pairs in range
-                    auto cand = [&](uint32_t k, const typename Vec4<R>::T& q, bool live) {
-                        R rx, ry, rz, r2, th;
-                        if (!F.in_range(q, bx, by, k < ks ? bz0 : bz1, rx, ry, rz, r2, th, live)) return;
-                        F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
-                               w.frec_sb[k]);
-                    };
-                    // kForceBatch position records in flight, also for a piece's last partial
-                    // group (its spare slots re-read the piece's last record, not in range)
-                    const uint32_t last = k1 - 1;
-                    for (uint32_t k = k0; k < k1; k += kForceBatch) {
-                        typename Vec4<R>::T q[kForceBatch];
+                    // In chunks of up to 64 candidates: screen the position records
+                    // (kForceBatch in flight; a chunk's last partial group re-reads its
+                    // last record, never in range) into a bit mask, then run the pair
+                    // body over the set bits only, in increasing k as before.  A wave's
+                    // body loop then takes as many turns as its lane with the most pairs
+                    // in range, not as its lane with the longest run.
+                    for (uint32_t c0 = k0; c0 < k1; c0 += 64u) {
+                        const uint32_t c1 = min(c0 + 64u, k1), last = c1 - 1;
+                        uint64_t mask = 0;
+                        for (uint32_t k = c0; k < c1; k += kForceBatch) {
+                            typename Vec4<R>::T q[kForceBatch];
 #pragma unroll
-                        for (int b = 0; b < kForceBatch; b++) q[b] = w.frec_pos[min(k + b, last)];
+                            for (int b = 0; b < kForceBatch; b++) q[b] = w.frec_pos[min(k + b, last)];
 #pragma unroll
-                        for (int b = 0; b < kForceBatch; b++) cand(min(k + b, last), q[b], k + b < k1);
+                            for (int b = 0; b < kForceBatch; b++) {
+                                R rx, ry, rz, r2, th;
+                                const bool in = F.in_range(q[b], bx, by, k + b < ks ? bz0 : bz1, rx, ry, rz, r2, th,
+                                                           k + b < c1);
+                                mask |= (uint64_t)in << (k + b - c0);
+                            }
+                        }
+                        while (mask) {
+                            const uint32_t k = c0 + (uint32_t)__builtin_ctzll(mask);
+                            mask &= mask - 1;
+                            R rx, ry, rz, r2, th;
+                            F.in_range(w.frec_pos[k], bx, by, k < ks ? bz0 : bz1, rx, ry, rz, r2, th);
+                            F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
+                                   w.frec_sb[k]);
+                        }
                     }
                 });
             }

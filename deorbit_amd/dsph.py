@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdsph.so")
 
-MIXED, FP64 = 0, 1
+MIXED, FP64, MIXED_FAST = 0, 1, 2
 (F_POS, F_VEL, F_ACC, F_MASS, F_RHO, F_U, F_DUDT, F_P, F_CS, F_CSIG, F_HH, F_EPS_WORK, F_D,
  F_EPS_ACT, F_SIGMAX, F_PF, F_RART, F_DSCALE, F_S, F_DSDT, F_MAT, F_FIXED, F_SEG, F_DUDT_OLD,
  F_DSDT_OLD) = range(25)

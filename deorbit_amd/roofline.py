@@ -68,6 +68,11 @@ def step_flops(iterations: int = 2) -> int:
     return PAIRS_PER_PARTICLE * (iterations * FLOPS_DENSITY_PAIR + FLOPS_FORCE_PAIR)
 
 
+def force_flops() -> int:
+    """Algorithmic flops of the force pass per particle."""
+    return PAIRS_PER_PARTICLE * FLOPS_FORCE_PAIR
+
+
 def _code_only(text: str) -> str:
     """C++ source without comments and with whitespace runs collapsed (string
     literals are kept as written)."""
@@ -91,10 +96,9 @@ def source_digest() -> str:
     return h.hexdigest()[:16]
 
 
-def recorded_traffic(folder: str, kernel: str, workload: str) -> dict | None:
-    """HBM bytes per launch recorded by ``rocpd_summary traffic`` (a ``traffic:``
-    line in any summary under `folder`) for this kernel, workload and source
-    digest; None when absent or recorded for other sources."""
+def _recorded(folder: str, tag: str, kernel: str, workload: str) -> dict | None:
+    """Key/value pairs of a ``<tag>:`` line in any summary under `folder` recorded
+    for this kernel, workload and the current source digest."""
     import glob
     import os
     import re
@@ -102,11 +106,39 @@ def recorded_traffic(folder: str, kernel: str, workload: str) -> dict | None:
     paths = sorted(glob.glob(os.path.join(folder, "*.md")) + glob.glob(os.path.join(folder, "*.txt")))
     for path in paths:
         with open(path) as f:
-            lines = [ln for ln in f if ln.startswith("traffic:")]
+            lines = [ln for ln in f if ln.startswith(tag + ":")]
         for line in lines:
             # values run to the next " key=" (kernel names hold spaces)
-            kv = dict(re.findall(r"(\w+)=(.*?)(?=\s+\w+=|\s*$)", line[len("traffic:"):].strip()))
+            kv = dict(re.findall(r"(\w+)=(.*?)(?=\s+\w+=|\s*$)", line[len(tag) + 1:].strip()))
             if kv.get("kernel") == kernel and kv.get("workload") == workload and kv.get("digest") == digest:
-                return {"bytes": float(kv["bytes"]), "fetch_factor": float(kv["fetch_factor"]),
-                        "write_factor": float(kv["write_factor"])}
+                return kv
     return None
+
+
+def recorded_traffic(folder: str, kernel: str, workload: str) -> dict | None:
+    """HBM bytes per launch recorded by ``rocpd_summary traffic`` (a ``traffic:``
+    line in any summary under `folder`) for this kernel, workload and source
+    digest; None when absent or recorded for other sources."""
+    kv = _recorded(folder, "traffic", kernel, workload)
+    if kv is None:
+        return None
+    return {"bytes": float(kv["bytes"]), "fetch_factor": float(kv["fetch_factor"]),
+            "write_factor": float(kv["write_factor"])}
+
+
+def recorded_pipes(folder: str, kernel: str, workload: str) -> dict | None:
+    """Pipe utilisation recorded by ``rocpd_summary pipes`` (VALU issue, lane
+    utilisation, TA and TD busy, L1 hit) for this kernel, workload and source
+    digest; None when absent or recorded for other sources."""
+    kv = _recorded(folder, "pipes", kernel, workload)
+    if kv is None:
+        return None
+    return {k: float(kv[k]) for k in ("valu_issue", "lane_util", "ta_busy", "td_busy", "l1_hit")}
+
+
+def limiter(pipes: dict | None) -> str:
+    """The busiest pipe of a kernel by its counters: "valu" (vector ALU issue) or
+    "vmem" (the vector memory path, texture address / data units), or "unmeasured"."""
+    if pipes is None:
+        return "unmeasured"
+    return "valu" if pipes["valu_issue"] >= max(pipes["ta_busy"], pipes["td_busy"]) else "vmem"

@@ -49,3 +49,55 @@ def gpu():
     from deorbit_amd import dsph
     dsph.load_library()
     return dsph
+
+
+P_FLOOR = 1.0e7      # Pa: per-particle pressure errors count where |P| exceeds this (eos_gpu_test.cpp:72-73)
+MELT_U = 1.0e6       # J/kg: the melt threshold of the reference driver's progress line
+
+
+def field_report(dev: dict, orc: dict, fields=("rho", "P", "Pf", "speed", "u", "D")):
+    """Per field, (max, RMS) of the scale-normalised error |g - c| / max|c|; and the
+    largest per-particle relative errors of rho, and of P where |P| > 10 MPa."""
+    out = {}
+    for f in fields:
+        g, c = np.asarray(dev[f], float), np.asarray(orc[f], float)
+        scale = float(np.abs(c).max()) if c.size else 0.0
+        if scale == 0.0:
+            out[f] = (float(np.abs(g).max()) if g.size else 0.0, 0.0)
+            continue
+        d = np.abs(g - c) / scale
+        out[f] = (float(d.max()), float(np.sqrt(np.mean(d * d))))
+    rho_rel = float(np.max(np.abs(dev["rho"] - orc["rho"]) / np.abs(orc["rho"])))
+    big = np.abs(orc["P"]) > P_FLOOR
+    p_rel = float(np.max(np.abs(dev["P"][big] - orc["P"][big]) / np.abs(orc["P"][big]))) if big.any() else 0.0
+    return out, rho_rel, p_rel
+
+
+def format_report(errs: dict) -> str:
+    return " | ".join(f"{f} {mx:.1e}/{rms:.1e}" for f, (mx, rms) in errs.items())
+
+
+def oracle_fields(o, po) -> dict:
+    v = o.get(po.VEL)
+    return {"rho": o.get(po.RHO), "P": o.get(po.P), "Pf": o.get(po.PF), "speed": np.linalg.norm(v, axis=1),
+            "u": o.get(po.U), "D": o.get(po.D), "pos": o.get(po.POS), "vel": v, "mass": o.get(po.MASS),
+            "hh": o.get(po.HH), "acc": o.get(po.ACC), "dudt": o.get(po.DUDT)}
+
+
+def device_fields(s) -> dict:
+    st = s.state(["rho", "P", "Pf", "vel", "u", "D", "pos", "mass", "hh", "acc", "dudt"])
+    st["speed"] = np.linalg.norm(st["vel"], axis=1)
+    return st
+
+
+def impactor_aggregates(st: dict, mat, imp_mat: int = 1):
+    """Mass-weighted impactor centroid, mean speed and vertical velocity, melt fraction
+    (u > 1 MJ/kg) and u max of the impactor: what the reference driver prints."""
+    imp = np.asarray(mat) == imp_mat
+    m = st["mass"][imp]
+    cen = (m[:, None] * st["pos"][imp]).sum(0) / m.sum()
+    spd = float((m * np.linalg.norm(st["vel"][imp], axis=1)).sum() / m.sum())
+    vz = float((m * st["vel"][imp][:, 2]).sum() / m.sum())
+    u = st["u"][imp]
+    return {"x": float(cen[0]), "z": float(cen[2]), "speed": spd, "vz": vz, "melt": float(np.mean(u > MELT_U)),
+            "umax": float(u.max())}

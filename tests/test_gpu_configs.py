@@ -1,0 +1,301 @@
+"""The device step on every benchmark workload at full size, against the oracle.
+
+* layered_4m (4.0M particles: iron sphere R = 60 dx at 2 deg, 8 km/s, into 32
+  water layers over 136 granite layers; strength and damage on): both device
+  paths after init and over 5 dt-lockstep steps, per field, against the oracle
+  run on the whole workload; the water layer (G = 0, no flaws) is reported on
+  its own.
+* ensemble_256 (256 impacts, 13.9M particles in one batch): after 3 batched
+  steps the smallest, median and largest instances equal separate runs bit for
+  bit, every segment keeps its momentum balance, h clamp and its own CFL step,
+  and in FP64 those three instances follow the oracle in dt lockstep.
+* halfspace_32m (32.2M particles, Gaussian relief, a mass-scaled top layer,
+  a four-pass radix sort): FP64 init against the oracle on the whole workload,
+  and invariants of the mixed path after a few steps.
+* graze_1m (1.05M particles): both paths over 20 dt-lockstep steps.
+* A free-running 107k-particle impact against a free-running oracle to the same
+  end time: the same number of steps and the same impactor aggregates, as the
+  reference's own long-run check of its GPU driver compares its production run
+  with the CPU code (README.gpu.md, item 6).
+
+Errors are scale-normalised |g - c| / max|c| per field unless named
+per-particle; the bars are written in each assertion.
+"""
+import numpy as np
+import pytest
+
+from conftest import (P_FLOOR, device_fields, field_report, format_report, impactor_aggregates, oracle_fields,
+                      scale_err)
+from deorbit_amd import dsph, ics
+from deorbit_amd import materials as M
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {dsph.FP64: "fp64", dsph.MIXED: "mixed"}
+FIELDS = ("pos", "vel", "u", "rho", "hh", "S", "D", "acc", "dudt")
+# mixed path, per field (rho, P, Pf, |v|, u; D separately): scale-normalised bars by
+# steps taken (FP32 pair sums round at ~1e-7; impacts amplify it as shocks steepen)
+MIXED_BAR = {0: 2e-5, 1: 2e-5, 5: 1e-4, 10: 2e-4, 20: 1e-3}
+# per particle, mixed path: density relative, and pressure relative where |P| > 10 MPa
+RHO_PARTICLE_BAR = 1e-4
+P_PARTICLE_BAR = 1e-4
+
+
+def check_mixed(tag, k, errs, rho_rel, p_rel):
+    worst = max(mx for f, (mx, _) in errs.items() if f != "D")
+    assert worst < MIXED_BAR[k], (tag, k, errs)
+    assert rho_rel < RHO_PARTICLE_BAR and p_rel < P_PARTICLE_BAR, (tag, k, rho_rel, p_rel)
+    assert errs["D"][1] < 1e-2, (tag, k, errs["D"])
+
+
+def report(tag, k, errs, rho_rel, p_rel):
+    print(f"| {tag} | {k} | {format_report(errs)} | rho rel {rho_rel:.1e} | P rel {p_rel:.1e} |", flush=True)
+
+
+def oracle_for(ic, nthreads=0):
+    return po.make_system(ic, ic["params"], ic["materials"], nthreads=nthreads,
+                          seed_damage=12345 if ic["params"]["damage_on"] else None, dx=ic["dx"])
+
+
+# ------------------------------------------------------------------ layered_4m
+@pytest.fixture(scope="module")
+def layered():
+    ic = ics.layered_4m()
+    print(f"\nlayered_4m: oracle init of {len(ic['mass'])} particles", flush=True)
+    o = oracle_for(ic)
+    o.init()
+    init = oracle_fields(o, po)
+    init["dSdt"], init["sigmax"] = o.get(po.DSDT), o.get(po.SIGMAX)
+    dts, states = [], {}
+    for k in range(1, 6):
+        dts.append(o.timestep())
+        o.step(dts[-1])
+        print(f"layered_4m: oracle step {k}", flush=True)
+        if k in (1, 5):
+            states[k] = oracle_fields(o, po)
+    del o
+    return ic, init, dts, states
+
+
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED], ids=["fp64", "mixed"])
+def test_layered_4m_against_the_oracle(layered, precision, gpu):
+    ic, init, dts, states = layered
+    tag = f"layered_4m {NAMES[precision]}"
+    water = ic["mat"] == M.MAT_WATER
+    assert water.sum() > 500_000 and len(ic["mass"]) > 3_900_000
+    s = dsph.make_system(ic, precision)
+    s.init()
+    dev = device_fields(s)
+    errs, rho_rel, p_rel = field_report(dev, init)
+    report(tag, 0, errs, rho_rel, p_rel)
+    w_errs, w_rho, w_p = field_report({k: v[water] for k, v in dev.items()}, {k: v[water] for k, v in init.items()})
+    report(tag + " water", 0, w_errs, w_rho, w_p)
+    if precision == dsph.FP64:
+        for f, g in (("acc", s.get("acc")), ("dudt", s.get("dudt")), ("dSdt", s.get("dSdt")),
+                     ("sigmax", s.get("sigmax")), ("hh", dev["hh"])):
+            e = scale_err(g, init[f])
+            print(f"{tag} init {f}={e:.2e}", flush=True)
+            assert e < 1e-12, f
+        assert max(mx for mx, _ in errs.values()) < 1e-12, errs
+    else:
+        check_mixed(tag, 0, errs, rho_rel, p_rel)
+        check_mixed(tag + " water", 0, w_errs, w_rho, w_p)
+    for k in range(1, 6):
+        s.step(dts[k - 1])
+        if k not in states:
+            continue
+        dev = device_fields(s)
+        errs, rho_rel, p_rel = field_report(dev, states[k])
+        report(tag, k, errs, rho_rel, p_rel)
+        if precision == dsph.FP64:
+            assert max(mx for mx, _ in errs.values()) < 1e-11, (k, errs)
+        else:
+            check_mixed(tag, k, errs, rho_rel, p_rel)
+    d = s.diagnostics()
+    assert d["status"] == 0 and d["nonfinite"] == 0 and d["steps"] == 5
+    s.close()
+
+
+# ------------------------------------------------------------------ ensemble_256
+def test_ensemble_256_batch(gpu):
+    members = ics.ensemble(256, 500)
+    sizes = np.array([len(m["mass"]) for m in members])
+    order = np.argsort(sizes, kind="stable")
+    chosen = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(off[-1])
+    assert n > 13_000_000 and sizes.min() >= 8_000 * 0.8 and sizes.max() <= 200_000 * 1.2
+    steps = 3
+    # mixed batch: three instances bit for bit as separate contexts
+    b = dsph.make_batch(members, dsph.MIXED)
+    b.init()
+    b.run(steps, 0.0, True)
+    dts = b.timestep()
+    assert len(dts) == 256 and np.all(dts > 0) and len(np.unique(dts)) > 200     # each segment its own step
+    batch = {f: b.get(f) for f in FIELDS}
+    mass = b.get("mass")
+    h0 = members[0]["params"]["h_init"]
+    for k in range(256):
+        d = b.diagnostics(k)
+        assert d["status"] == 0 and d["nonfinite"] == 0 and d["n"] == sizes[k] and d["steps"] == steps, (k, d)
+        sl = slice(off[k], off[k + 1])
+        ma = mass[sl, None] * batch["acc"][sl]
+        assert np.all(np.abs(ma.sum(0)) <= 2e-5 * np.abs(ma).sum(0)), k
+        h = batch["hh"][sl]
+        assert h.min() >= 0.5 * h0 * (1 - 1e-6) and h.max() <= 2 * h0 * (1 + 1e-6), k
+    b.close()
+    for k in chosen:
+        s = dsph.make_system(members[k], dsph.MIXED)
+        s.init()
+        s.run(steps, 0.0, True)
+        sl = slice(off[k], off[k + 1])
+        for f in FIELDS:
+            assert np.array_equal(s.get(f), batch[f][sl]), (k, f)
+        s.close()
+    print(f"ensemble_256: {n} particles, instances {chosen} (sizes {sizes[chosen].tolist()}) bitwise equal to "
+          f"separate runs after {steps} batched steps", flush=True)
+    del batch
+    # FP64 batch against the oracle on the same three instances, dt lockstep
+    b = dsph.make_batch(members, dsph.FP64)
+    b.init()
+    orcs = {k: oracle_for(members[k]) for k in chosen}
+    for o in orcs.values():
+        o.init()
+    pairs = (("rho", po.RHO), ("hh", po.HH), ("P", po.P), ("acc", po.ACC), ("u", po.U), ("vel", po.VEL),
+             ("pos", po.POS), ("D", po.D))
+    for step in range(steps + 1):
+        got = {f: b.get(f) for f, _ in pairs}
+        for k, o in orcs.items():
+            sl = slice(off[k], off[k + 1])
+            for f, pf in pairs:
+                e = scale_err(got[f][sl], o.get(pf))
+                assert e < (1e-12 if step == 0 else 1e-10), (k, step, f, e)
+        if step == steps:
+            break
+        dts = b.timestep()
+        for k, o in orcs.items():
+            dts[k] = o.timestep()
+            o.step(dts[k])
+        b.step_dts(dts)
+    b.close()
+
+
+# ------------------------------------------------------------------ halfspace_32m
+def test_halfspace_32m(gpu):
+    ic = ics.halfspace_32m()
+    n = len(ic["mass"])
+    assert n > 2 ** 24 and n > 32_000_000
+    # FP64 init on the whole workload against the oracle
+    print(f"\nhalfspace_32m: oracle init of {n} particles", flush=True)
+    o = oracle_for(ic)
+    o.init()
+    print("halfspace_32m: oracle init done", flush=True)
+    s = dsph.make_system(ic, dsph.FP64)
+    s.init()
+    for name, f in (("rho", po.RHO), ("hh", po.HH), ("P", po.P), ("acc", po.ACC), ("dudt", po.DUDT)):
+        e = scale_err(s.get(name), o.get(f))
+        print(f"halfspace_32m init fp64 {name}={e:.2e}", flush=True)
+        assert e < 1e-12, name
+    assert abs(s.timestep() - o.timestep()) <= 1e-14 * o.timestep()
+    s.close()
+    rho_o = o.get(po.RHO)
+    del o
+    # mixed path: init against the same oracle state, then invariants over a few steps
+    s = dsph.make_system(ic, dsph.MIXED)
+    s.init()
+    rho = s.get("rho")
+    e = float(np.max(np.abs(rho - rho_o) / rho_o))
+    print(f"halfspace_32m init mixed rho per particle={e:.2e}", flush=True)
+    assert e < RHO_PARTICLE_BAR
+    # the mass-scaled top layer (0.98 rho0 dx^3 particles) reads 0.98 of the density below it
+    m0 = M.impact_materials(True)[M.MAT_TARGET]["rho0"] * ic["dx"] ** 3
+    top = (ic["mat"] == M.MAT_TARGET) & (ic["mass"] < 0.99 * m0)
+    deep = (ic["mat"] == M.MAT_TARGET) & (ic["pos"][:, 2] < -100.0 * ic["dx"])
+    ratio = float(np.median(rho[top]) / np.median(rho[deep]))
+    print(f"halfspace_32m top-layer density ratio {ratio:.5f}", flush=True)
+    assert abs(ratio - 0.98) < 2e-3
+    h0 = ic["params"]["h_init"]
+    for k in range(2):
+        m = s.get("mass")
+        ma = m[:, None] * s.get("acc")
+        assert np.all(np.abs(ma.sum(0)) <= 2e-5 * np.abs(ma).sum(0)), k
+        h = s.get("hh")
+        assert h.min() >= 0.5 * h0 * (1 - 1e-6) and h.max() <= 2 * h0 * (1 + 1e-6)
+        d = s.diagnostics()
+        assert d["status"] == 0 and d["nonfinite"] == 0 and d["n"] == n
+        s.run(3, 0.0, True)
+    s.close()
+
+
+# ------------------------------------------------------------------ graze_1m
+def test_graze_1m_twenty_lockstep_steps(gpu):
+    ic = ics.graze_1m()
+    o = oracle_for(ic)
+    o.init()
+    sys_ = {p: dsph.make_system(ic, p) for p in (dsph.FP64, dsph.MIXED)}
+    for s in sys_.values():
+        s.init()
+    for k in range(1, 21):
+        dt = o.timestep()
+        o.step(dt)
+        for s in sys_.values():
+            s.step(dt)
+        if k not in (1, 10, 20):
+            continue
+        orc = oracle_fields(o, po)
+        for p, s in sys_.items():
+            errs, rho_rel, p_rel = field_report(device_fields(s), orc)
+            report(f"graze_1m {NAMES[p]}", k, errs, rho_rel, p_rel)
+            if p == dsph.FP64:
+                assert max(mx for mx, _ in errs.values()) < (1e-12 if k == 1 else 1e-10), (k, errs)
+            else:
+                check_mixed("graze_1m mixed", k, errs, rho_rel, p_rel)
+    for s in sys_.values():
+        assert s.diagnostics()["status"] == 0
+        s.close()
+
+
+# ------------------------------------------------------------------ free-running
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED], ids=["fp64", "mixed"])
+def test_free_running_impact_matches_a_free_running_oracle(precision, gpu):
+    """Both codes choose their own CFL steps (sph.hpp:350-355) up to a common end
+    time, 200.5 of the oracle's steps: the same 201 steps (the last one clamped to
+    the end time, run_ic.cpp:327-330) and the impactor aggregates the driver
+    prints (centroid x and z, mean |v| and v_z, melt fraction, u max)."""
+    ic = ics.impact(20, 30.0, 8000.0, 64, 48, [(M.MAT_TARGET, 24)], seed=1200, strength=True, frac=True,
+                    name="impact_107k", steps=200)
+    n = len(ic["mass"])
+    assert n > 100_000
+    o = oracle_for(ic)
+    o.init()
+    t = 0.0
+    for _ in range(200):
+        dt = o.timestep()
+        o.step(dt)
+        t += dt
+    t_end = t + 0.5 * o.timestep()
+    o.step(t_end - t)
+    orc = oracle_fields(o, po)
+    prm = dict(ic["params"], t_end=t_end)
+    s = dsph.System(prm, ic["materials"], precision)
+    s.add_ic(ic)
+    s.seed_damage(ic["dx"], 12345)
+    s.init()
+    steps = 0
+    while s.diagnostics()["t"] < t_end and steps < 400:
+        s.run(1, 0.0, False)
+        steps += 1
+    d = s.diagnostics()
+    assert d["t"] == t_end and d["status"] == 0
+    a_dev = impactor_aggregates(device_fields(s), ic["mat"])
+    a_orc = impactor_aggregates(orc, ic["mat"])
+    print(f"free-running {NAMES[precision]}: {steps} steps (oracle 201) to t={t_end:.6e} s; " +
+          " ".join(f"{k} {a_dev[k]:.6g}/{a_orc[k]:.6g}" for k in a_dev), flush=True)
+    assert steps == 201
+    rel = 1e-9 if precision == dsph.FP64 else 1e-5
+    for k in ("x", "z", "speed", "umax"):
+        assert abs(a_dev[k] - a_orc[k]) <= rel * max(abs(a_orc[k]), 1.0), (k, a_dev[k], a_orc[k])
+    assert abs(a_dev["vz"] - a_orc["vz"]) <= rel * a_orc["speed"]
+    assert abs(a_dev["melt"] - a_orc["melt"]) <= (0.0 if precision == dsph.FP64 else 2e-3)
+    s.close()

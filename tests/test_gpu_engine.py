@@ -52,7 +52,7 @@ def test_graph_replay_equals_stream_launches(precision, gpu):
     assert da["t"] == db["t"] and da["steps"] == db["steps"] == 7
 
 
-@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED])
+@pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED_FAST])
 def test_neighbour_lists_give_the_cell_walk_results(precision, gpu):
     # The lists hold the walk's pairs in the walk's order, so every way of
     # falling back (lists off, lists overflowing, h outgrowing a tiny skin)
@@ -124,9 +124,10 @@ def test_tuning_changes_speed_not_results_beyond_roundoff(cell_factor, gpu):
 
 def test_tuning_limits_are_enforced(gpu):
     """Columns narrower than h_init or a list skin above 0.25 could let a z range
-    reach past two slice blocks; the library refuses them."""
+    reach past two slice blocks, and neighbour lists serve only the FP32 density
+    sums; the library refuses them."""
     s = dsph.make_system(ics.tiny_impact(), dsph.MIXED)
-    for bad in (dict(cell_factor=0.5), dict(list_skin=0.5)):
+    for bad in (dict(cell_factor=0.5), dict(list_skin=0.5), dict(neighbor_capacity=128)):
         with pytest.raises(dsph.DsphError) as e:
             s.set_tuning(**bad)
         assert e.value.code == -1

@@ -150,7 +150,16 @@ def test_density_and_pressure_per_particle(precision, gpu):
     e_p = rel_err(s.get("P"), o.get(po.P), floor=1e7)
     report(f"per-particle {NAMES[precision]}", {"rho": e_rho, "P": e_p})
     assert e_rho < (1e-13 if precision == dsph.FP64 else 1e-4)
-    assert e_p < (1e-11 if precision == dsph.FP64 else 1e-3)
+    assert e_p < (1e-11 if precision == dsph.FP64 else 1e-4)
+    if precision == dsph.MIXED:
+        # the FP32 density sums of MIXED_FAST keep density within 1e-4 per particle,
+        # but not the pressure near rho0 (the reference's FP32 EOS gate, eos_gpu_test.cpp:81)
+        f = dsph.make_system(ic, dsph.MIXED_FAST)
+        f.init()
+        e_rho_f, e_p_f = rel_err(f.get("rho"), o.get(po.RHO)), rel_err(f.get("P"), o.get(po.P), floor=1e7)
+        report("per-particle mixed_fast", {"rho": e_rho_f, "P": e_p_f})
+        assert e_rho_f < 1e-4 and e_p_f < 5e-3
+        f.close()
 
 
 def test_eos_sweep_on_device(gpu):

@@ -36,8 +36,10 @@ def lockstep(ic, precision, steps, tuning=None):
     return errs
 
 
-@pytest.mark.parametrize("precision,tol", [(dsph.FP64, 1e-12), (dsph.MIXED, 2e-5)])
-@pytest.mark.parametrize("lists", [-1, 128])
+# neighbour lists (128 entries) serve the FP64 path and the FP32 density sums of MIXED_FAST
+@pytest.mark.parametrize("precision,tol,lists", [(dsph.FP64, 1e-12, -1), (dsph.FP64, 1e-12, 128),
+                                                 (dsph.MIXED, 2e-5, -1), (dsph.MIXED_FAST, 2e-5, -1),
+                                                 (dsph.MIXED_FAST, 2e-5, 128)])
 def test_periodic_sod_small_matches_oracle(precision, tol, lists):
     # 11,520 particles, eight of the right state across the periodic width (with
     # four, the particle half a period away sits within 2h and its minimum image
