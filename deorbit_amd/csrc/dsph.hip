@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <tuple>
@@ -1088,6 +1089,106 @@ class Engine : public EngineBase {
 
 }  // namespace dsph
 
+// ============================================================================ allocation-free sums
+// Compensated FP64 mass, energy and momentum sums of caller-owned device arrays
+// without any allocation: the per-chunk partials live in `scratch`, a
+// file-scope device array of fixed size, one copy per GPU.  Host code never
+// passes `scratch` itself to a kernel: its device address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch.
+namespace dsph {
+
+constexpr int kSumsChunk = 16384;          // particles per partial (the chunk of dsph_diagnostics)
+constexpr int kSumsMaxChunks = 4096;       // partials the scratch array holds
+constexpr int64_t kSumsMaxN = (int64_t)kSumsChunk * kSumsMaxChunks;   // 67,108,864 particles
+constexpr int kSumsOut = 8;                // mass, e_kin, e_int, e_tot, mom x, y, z, non-finite count
+
+struct SumsPart {
+    KSum sum[kDiagSums];                   // mass, e_kin, e_int, mom x, y, z
+    double nonfinite;
+};
+
+static __device__ __align__(256) SumsPart scratch[kSumsMaxChunks];   // 416 KiB per GPU
+
+// One block per chunk of kSumsChunk particles: per-thread compensated sums over
+// a strided share, then a fixed-order tree over the block (as k_diag_partial).
+// Particles with a non-finite mass, velocity or u are skipped and counted.
+__global__ __launch_bounds__(256) void k_sums_partial(const double* m, const double* vx, const double* vy,
+                                                      const double* vz, const double* u, int64_t n,
+                                                      SumsPart* parts) {
+    __shared__ SumsPart sh[256];
+    const int64_t beg = (int64_t)blockIdx.x * kSumsChunk;
+    const int64_t end = beg + kSumsChunk < n ? beg + kSumsChunk : n;
+    SumsPart acc;
+    for (int k = 0; k < kDiagSums; k++) acc.sum[k] = KSum{0.0, 0.0};
+    acc.nonfinite = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+        const double mi = m[i], x = vx[i], y = vy[i], z = vz[i], ui = u[i];
+        const double v2 = x * x + y * y + z * z;
+        if (!(isfinite(mi) && isfinite(v2) && isfinite(ui))) { acc.nonfinite += 1.0; continue; }
+        ksum_add(acc.sum[0], mi);
+        ksum_add(acc.sum[1], 0.5 * mi * v2);
+        ksum_add(acc.sum[2], mi * ui);
+        ksum_add(acc.sum[3], mi * x);
+        ksum_add(acc.sum[4], mi * y);
+        ksum_add(acc.sum[5], mi * z);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            SumsPart a = sh[threadIdx.x];
+            const SumsPart& b = sh[threadIdx.x + off];
+            for (int k = 0; k < kDiagSums; k++) a.sum[k] = ksum_merge(a.sum[k], b.sum[k]);
+            a.nonfinite += b.nonfinite;
+            sh[threadIdx.x] = a;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = sh[0];
+}
+
+// One thread folds the chunk partials in chunk order and writes the results.
+__global__ void k_sums_final(const SumsPart* parts, int nchunks, double* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    SumsPart acc;
+    for (int k = 0; k < kDiagSums; k++) acc.sum[k] = KSum{0.0, 0.0};
+    acc.nonfinite = 0.0;
+    for (int q = 0; q < nchunks; q++) {
+        for (int k = 0; k < kDiagSums; k++) acc.sum[k] = ksum_merge(acc.sum[k], parts[q].sum[k]);
+        acc.nonfinite += parts[q].nonfinite;
+    }
+    KSum tot = acc.sum[1];
+    ksum_add(tot, acc.sum[2].s);
+    tot.c += acc.sum[2].c;
+    out[0] = acc.sum[0].s + acc.sum[0].c;
+    out[1] = acc.sum[1].s + acc.sum[1].c;
+    out[2] = acc.sum[2].s + acc.sum[2].c;
+    out[3] = tot.s + tot.c;
+    for (int a = 0; a < 3; a++) out[4 + a] = acc.sum[3 + a].s + acc.sum[3 + a].c;
+    out[7] = acc.nonfinite;
+}
+
+static std::mutex g_scratch_mu;
+static std::map<int, SumsPart*> g_scratch;    // device ordinal -> device address of `scratch`
+
+// Device address of `scratch` on the current device, looked up on the first
+// use of each device and kept.
+static SumsPart* scratch_on_current_device() {
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch.find(dev);
+    if (it != g_scratch.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        throw Err{DSPH_E_HIP, std::string("hipGetSymbolAddress of the sums scratch failed: ") + hipGetErrorString(e)};
+    g_scratch[dev] = (SumsPart*)ptr;
+    return (SumsPart*)ptr;
+}
+
+}  // namespace dsph
+
 // ============================================================================ C ABI
 using namespace dsph;
 
@@ -1172,6 +1273,7 @@ int create_impl(const dsph_params* params, int32_t nseg, const dsph_material* ma
     require(ndev > 0, DSPH_E_HIP, "no HIP device");
     auto c = std::make_unique<dsph_ctx>();
     HIPC(hipGetDevice(&c->device));
+    (void)scratch_on_current_device();      // set-up of the allocation-free sums on this device
     HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->precision = precision;
     c->segp.assign(params, params + nseg);
@@ -1345,6 +1447,27 @@ int dsph_set_tuning(dsph_ctx* c, const dsph_tuning* t) {
 }
 int dsph_autotune(dsph_ctx* c, int32_t steps, dsph_tuning* chosen) {
     CTX_OP(require(steps >= 1, DSPH_E_ARG, "steps must be >= 1"); e.autotune(steps, chosen));
+}
+
+int dsph_energy_sums_noalloc(const double* d_mass, const double* d_vx, const double* d_vy, const double* d_vz,
+                             const double* d_u, int64_t n, void* stream, double* d_out) {
+    return guarded([&] {
+        require(d_mass && d_vx && d_vy && d_vz && d_u && d_out, DSPH_E_ARG, "null device pointer");
+        require(n >= 0, DSPH_E_ARG, "n must be non-negative");
+        require(n <= kSumsMaxN, DSPH_E_RANGE,
+                "n = " + std::to_string(n) + " exceeds the " + std::to_string(kSumsMaxN) +
+                    " particles the allocation-free sums hold (" + std::to_string(kSumsMaxChunks) + " chunks of " +
+                    std::to_string(kSumsChunk) + ")");
+        SumsPart* parts = scratch_on_current_device();
+        const int nchunks = n > 0 ? (int)((n + kSumsChunk - 1) / kSumsChunk) : 1;
+        hipLaunchKernelGGL(k_sums_partial, dim3(nchunks), dim3(256), 0, (hipStream_t)stream, d_mass, d_vx, d_vy, d_vz,
+                           d_u, n, parts);
+        check_launch("sums_partial");
+        hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(1), 0, (hipStream_t)stream, (const SumsPart*)parts, nchunks,
+                           d_out);
+        check_launch("sums_final");
+        return DSPH_OK;
+    });
 }
 
 }  // extern "C"

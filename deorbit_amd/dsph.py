@@ -35,7 +35,7 @@ EXPORTS = ["dsph_version", "dsph_last_error", "dsph_create", "dsph_batch_create"
            "dsph_init", "dsph_compute_density_h", "dsph_compute_forces", "dsph_grow_damage",
            "dsph_timestep", "dsph_step", "dsph_step_dts", "dsph_run", "dsph_synchronize", "dsph_set_clock",
            "dsph_diagnostics", "dsph_set_profiling", "dsph_get_profile", "dsph_reset_profile",
-           "dsph_get_tuning", "dsph_set_tuning", "dsph_autotune"]
+           "dsph_get_tuning", "dsph_set_tuning", "dsph_autotune", "dsph_energy_sums_noalloc"]
 
 
 class Material(C.Structure):
@@ -122,6 +122,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.dsph_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.dsph_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.dsph_autotune.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
+    lib.dsph_energy_sums_noalloc.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp, vp]
     _LIB = lib
     return lib
 

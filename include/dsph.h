@@ -198,6 +198,25 @@ int dsph_set_tuning(dsph_ctx* ctx, const dsph_tuning* t);
  * returned and applied. */
 int dsph_autotune(dsph_ctx* ctx, int32_t steps, dsph_tuning* chosen);
 
+/* ---- allocation-free sums -------------------------------------------- */
+/* Compensated (TwoSum) FP64 sums over n particles of caller-owned device
+ * arrays, enqueued on `stream` (a hipStream_t; NULL = the null stream) without
+ * synchronising: d_out[0..7] (device memory) receives total mass, kinetic
+ * energy sum(m v^2 / 2), internal energy sum(m u), total energy, momentum x, y,
+ * z, and the number of particles skipped for a non-finite m, v or u.  The same
+ * sums as dsph_diagnostics, over arrays instead of a context.
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  It allocates nothing and clears nothing: its per-chunk
+ * partials live in a fixed device array of 4096 chunks of 16384 particles
+ * (416 KiB per GPU, held until the process ends), so n is limited to
+ * 67,108,864 (larger n returns DSPH_E_RANGE); every call on one GPU shares that
+ * array, so calls that overlap on one GPU race.  The array's address on a
+ * device is looked up by dsph_create on that device, or else by the first call
+ * on that device (make that first call outside graph capture). */
+int dsph_energy_sums_noalloc(const double* d_mass, const double* d_vx, const double* d_vy, const double* d_vz,
+                             const double* d_u, int64_t n, void* stream, double* d_out);
+
 #ifdef __cplusplus
 }
 #endif
