@@ -62,12 +62,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_run(sysm, steps, graph=True):
+def timed_run(sysm, steps, chunk=50):
+    """Wall time of `steps` steps, enqueued `chunk` steps per dsph_run call (each call
+    waits for its steps and returns the device flags), so a long run never queues
+    more than a few thousand kernels ahead of the device."""
     if steps <= 0:
         return 0.0
     sysm.synchronize()
     t0 = time.perf_counter()
-    sysm.run(steps, 0.0, graph)
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        sysm.run(k, 0.0, GRAPH)
+        done += k
     sysm.synchronize()
     return time.perf_counter() - t0
 
@@ -80,6 +87,9 @@ def profile_leg(s, steps):
     prof = s.profile()
     s.set_profiling(False)
     return prof
+
+
+GRAPH = True     # steps replayed from a captured hipGraph (--no-graph: stream launches)
 
 
 def gpu_leg(ic, precision, preroll, warmup, steps, run_steps, autotune=True, profile_steps=0):
@@ -212,11 +222,16 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-fast", action="store_true", help="skip the DSPH_MIXED_FAST leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step's kernels on the stream instead of replaying a hipGraph "
+                         "(for tracing tools)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the library's default tuning instead of the autotuner's choice")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (counter calibration)")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = not args.no_graph
     if args.copy_probe:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -258,35 +273,55 @@ def main():
     for k, (tot, cnt) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         if cnt:
             log(f"  {k:14s} {tot / args.profile_steps:8.3f} ms/step  ({cnt} launches)")
-    kb = roofline.kernel_bytes("mixed")
-    # dominant kernel: the force pass, over neighbour lists or by cell walks
-    # (whichever the tuning selected)
-    phase = "force_list" if prof.get("force_list", (0.0, 0))[0] > prof["force"][0] else "force"
-    # (the benchmark workloads have no periodic axis: the non-periodic kernel variant)
-    kname = "k_force_list<float, false>" if phase == "force_list" else "k_force<float, false>"
-    force_ms = prof[phase][0] / max(1, prof[phase][1])
-    achieved = kb["force"] * n / (force_ms * 1e-3) / 1e9
-    force_tflops = roofline.force_flops() * n / (force_ms * 1e-3) / 1e12
-    # HBM bytes per launch and pipe utilisation from the committed rocprof summaries
-    # of this kernel and workload, used only when recorded for these device sources
+    # the two pair kernels against their rooflines; the dominant one (time per step)
+    # is the line's roofline.  Algorithmic bytes and flops per particle and step
+    # (deorbit_amd/roofline.py); the density kernel runs once per possible round
+    # (4 launches per step, usually one active), so its figures are per step:
+    # bytes per step / time per step = bytes per launch / mean launch time.
     pdir = os.path.join(ROOT, "profiles")
-    rec = roofline.recorded_traffic(pdir, kname, args.workload)
-    pipes = roofline.recorded_pipes(pdir, kname, args.workload)
-    roof = {"bound": roofline.limiter(pipes), "kernel": kname, "achieved": round(achieved, 2),
-            "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
-            "frac": round(achieved / roofline.MI355X_HBM_GBPS, 5),
-            "traffic": None if rec is None else rec["bytes"],
-            "bytes_per_particle": kb["force"], "algorithmic_bytes": kb["force"] * n,
-            "mean_launch_ms": round(force_ms, 4), "launches": prof[phase][1],
-            "source_digest": roofline.source_digest(), "pipes": pipes,
+    ps = args.profile_steps
+    # the force pass over neighbour lists or by cell walks (whichever the tuning
+    # selected); the benchmark workloads have no periodic axis: non-periodic variants
+    fphase = "force_list" if prof.get("force_list", (0.0, 0))[0] > prof["force"][0] else "force"
+    kernels = {
+        "force": dict(kernel="k_force_list<float, false>" if fphase == "force_list" else "k_force<float, false>",
+                      phase=fphase, bytes_pp=roofline.kernel_bytes("mixed")["force"], flops_pp=roofline.force_flops()),
+        "density": dict(kernel="k_density_round_hp<false>", phase="density",
+                        bytes_pp=roofline.density_round_bytes("mixed", True), flops_pp=roofline.density_flops()),
+    }
+    for key, k in kernels.items():
+        tot, launches = prof[k["phase"]]
+        step_ms = tot / ps
+        achieved = k["bytes_pp"] * n / (step_ms * 1e-3) / 1e9
+        tflops = k["flops_pp"] * n / (step_ms * 1e-3) / 1e12
+        # HBM bytes per launch and pipe utilisation from the committed rocprof summaries
+        # of this kernel and workload, used only when recorded for these device sources
+        rec = roofline.recorded_traffic(pdir, k["kernel"], args.workload)
+        pipes = roofline.recorded_pipes(pdir, k["kernel"], args.workload)
+        lps = launches / ps
+        k.update(step_ms=round(step_ms, 4), launches_per_step=lps, mean_launch_ms=round(tot / max(1, launches), 4),
+                 achieved=round(achieved, 2), frac=round(achieved / roofline.MI355X_HBM_GBPS, 5),
+                 traffic=None if rec is None else rec["bytes"], pipes=pipes, bound=roofline.limiter(pipes),
+                 valu_tflops=round(tflops, 3), valu_frac=round(tflops / roofline.MI355X_FP32_TFLOPS, 5))
+    dom = max(kernels, key=lambda key: kernels[key]["step_ms"])
+    kd = kernels[dom]
+    # per launch: algorithmic bytes (bytes per step / launches per step) over the mean launch
+    # time give the same rate as per step; traffic is the measured HBM bytes per launch
+    roof = {"bound": kd["bound"], "kernel": kd["kernel"], "achieved": kd["achieved"],
+            "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s", "frac": kd["frac"], "traffic": kd["traffic"],
+            "algorithmic_bytes": kd["bytes_pp"] * n / kd["launches_per_step"],
+            "bytes_per_particle_step": kd["bytes_pp"],
+            "launches_per_step": kd["launches_per_step"], "mean_launch_ms": kd["mean_launch_ms"],
+            "source_digest": roofline.source_digest(), "pipes": kd["pipes"],
+            "valu": {"flops_per_particle_step": kd["flops_pp"], "achieved_tflops": kd["valu_tflops"],
+                     "peak_tflops": roofline.MI355X_FP32_TFLOPS, "frac": kd["valu_frac"]},
             "step_frac": round(roofline.step_bytes("mixed") * value / 1e9 / roofline.MI355X_HBM_GBPS, 5),
-            # the kernel against the vector ALU: algorithmic FP32 pair flops per launch
-            "valu": {"flops_per_particle": roofline.force_flops(), "achieved_tflops": round(force_tflops, 3),
-                     "peak_tflops": roofline.MI355X_FP32_TFLOPS,
-                     "frac": round(force_tflops / roofline.MI355X_FP32_TFLOPS, 5)},
-            "step_valu": {"flops_per_particle_step": roofline.step_flops(),
-                          "achieved_tflops": round(roofline.step_flops() * value / 1e12, 3),
-                          "frac": round(roofline.step_flops() * value / 1e12 / roofline.MI355X_FP32_TFLOPS, 5)}}
+            "step_valu_frac": round(roofline.step_flops() * value / 1e12 / roofline.MI355X_FP32_TFLOPS, 5),
+            "kernels": {key: {f: k[f] for f in ("kernel", "step_ms", "launches_per_step", "mean_launch_ms",
+                                                "bytes_pp", "achieved", "frac", "traffic", "bound", "pipes",
+                                                "valu_tflops", "valu_frac")} for key, k in kernels.items()}}
+    log(f"roofline: dominant {kd['kernel']} {kd['step_ms']:.3f} ms/step, {kd['achieved']:.0f} GB/s algorithmic "
+        f"({100 * kd['frac']:.2f} % of HBM), {kd['valu_tflops']:.1f} TFLOP/s; bound {kd['bound']}")
     diag = s.diagnostics()
     s.close()
 
@@ -335,7 +370,7 @@ def main():
         "dtype": "f32 force pass, stress, sound speed; f64 density terms, pressure, state, reductions",
         "data": "synthetic (seeded lattice impact IC)",
         "config": {"workload": WORKLOAD[args.workload], "particles": n, "dt": "device CFL (cfl=0.2)",
-                   "graph": True, "window": f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps}"
+                   "graph": GRAPH, "window": f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps}"
                    f" of {cfg_steps} (autotuned at step {preroll})" if tune else
                    f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps} of {cfg_steps}",
                    "t_sim_end": diag["t"]},

@@ -704,7 +704,7 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
 // ~3e-7, which the second iteration's density passes on to P at up to ~1e-4
 // of |P| near 10 MPa.  Neighbour lists are not used.
 template <bool PER>
-__global__ __launch_bounds__(256) void k_density_round_hp(Carried<float> c, Work<float> w, int r, int remap,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_density_round_hp(Carried<float> c, Work<float> w, int r, int remap,
                                                          SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;

@@ -36,6 +36,13 @@ def kernel_bytes(precision: str = "mixed") -> dict:
     out["density"] = rec + r + 4 + 4 + d + 2 * r
     # last iteration adds the EOS epilogue: R u, S6, D, vel; W P cs csig Pf dscale Rart sigmax + 5 records
     out["density_last"] = out["density"] + (d + 7 * r + 3 * d) + (7 * r + 5 * rec)
+    if precision == "mixed":
+        # DSPH_MIXED's FP64-term density: the FP32 screening record and the FP64 (x, y, z, m)
+        # record of each neighbour once, the own position in FP64 besides the FP32 iteration's reads
+        out["density64"] = out["density"] + 32 + 3 * d
+        out["density64_last"] = out["density_last"] + 32 + 3 * d
+        # the permutation also writes the FP64 records
+        out["permute64"] = out["permute"] + 32
     # force: 5 neighbour records once + own (cellc, info, S6, dscale) R; W acc3 dudt dSdt6 eps_work
     out["force"] = 5 * rec + 4 + 4 + 6 * r + r + (3 * r + r + 6 * r + r)
     if precision != "mixed":
@@ -46,11 +53,20 @@ def kernel_bytes(precision: str = "mixed") -> dict:
     return out
 
 
-def step_bytes(precision: str = "mixed", passes: int = 3, iterations: int = 2) -> int:
-    """Algorithmic bytes of one full step (one density round of `iterations`)."""
+def density_round_bytes(precision: str = "mixed", density64: bool = True, iterations: int = 2) -> int:
+    """Algorithmic bytes of one density round (`iterations` iterations, EOS epilogue in the last)."""
     k = kernel_bytes(precision)
-    return (k["kick_drift"] + k["keys"] + passes * k["sort_pass"] + k["permute"]
-            + (iterations - 1) * k["density"] + k["density_last"] + k["force"] + k["finish"])
+    tag = "density64" if precision == "mixed" and density64 else "density"
+    return (iterations - 1) * k[tag] + k[tag + "_last"]
+
+
+def step_bytes(precision: str = "mixed", passes: int = 3, iterations: int = 2, density64: bool = True) -> int:
+    """Algorithmic bytes of one full step (one density round of `iterations`).  density64:
+    DSPH_MIXED's FP64 density records (False: DSPH_MIXED_FAST)."""
+    k = kernel_bytes(precision)
+    d64 = precision == "mixed" and density64
+    return (k["kick_drift"] + k["keys"] + passes * k["sort_pass"] + (k["permute64"] if d64 else k["permute"])
+            + density_round_bytes(precision, density64, iterations) + k["force"] + k["finish"])
 
 
 # Algorithmic floating-point work of one particle-step: the ordered neighbour
@@ -71,6 +87,11 @@ def step_flops(iterations: int = 2) -> int:
 def force_flops() -> int:
     """Algorithmic flops of the force pass per particle."""
     return PAIRS_PER_PARTICLE * FLOPS_FORCE_PAIR
+
+
+def density_flops(iterations: int = 2) -> int:
+    """Algorithmic flops of one density round per particle (FP64 in DSPH_MIXED)."""
+    return PAIRS_PER_PARTICLE * iterations * FLOPS_DENSITY_PAIR
 
 
 def _code_only(text: str) -> str:
