@@ -39,6 +39,10 @@ struct SegPar {
 // (h <= 2 h_init), meets at most two blocks as long as the tuning keeps the
 // column width >= h_init and the skin <= 0.25: 2 * 5 / 0.25 + 3 <= 48 + 1.
 constexpr int kSliceBlock = 48;
+// Zeroed records past the last particle in the density and force position
+// record arrays: the screens read candidates in groups of four without
+// clamping the group to its run (slots past the run never count).
+constexpr int kRecordPad = 4;
 constexpr double kSliceRatio = 1.0 / 4.0;
 __device__ __host__ __forceinline__ int block_start(int cz) { return cz - cz % kSliceBlock; }
 // z of a block's frame origin in slices: its middle, so FP32 offsets from it

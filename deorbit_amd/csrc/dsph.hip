@@ -146,9 +146,11 @@ class Engine : public EngineBase {
         w.eps_work = dalloc<R>(n_);
         w.dudt = dalloc<R>(n_); w.dSdt = dalloc<R>(6 * n_);
         w.dudt_old = dalloc<R>(n_); w.dSdt_old = dalloc<R>(6 * n_);
-        w.dens_rec = dalloc<V4>(n_);
+        w.dens_rec = dalloc<V4>(n_ + kRecordPad);
         w.dens_rec64 = density64_ ? dalloc<double4>(n_) : nullptr;
-        w.frec_pos = dalloc<V4>(n_); w.frec_vel = dalloc<V4>(n_); w.frec_th = dalloc<V4>(n_);
+        w.frec_pos = dalloc<V4>(n_ + kRecordPad); w.frec_vel = dalloc<V4>(n_); w.frec_th = dalloc<V4>(n_);
+        HIPC(hipMemsetAsync(w.dens_rec + n_, 0, kRecordPad * sizeof(V4), stream_));
+        HIPC(hipMemsetAsync(w.frec_pos + n_, 0, kRecordPad * sizeof(V4), stream_));
         w.frec_sa = dalloc<V4>(n_); w.frec_sb = dalloc<V4>(n_);
         w.cellc = dalloc<uint32_t>(n_);
         w.dev_of = dalloc<uint32_t>(n_);

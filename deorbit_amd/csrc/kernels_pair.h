@@ -295,7 +295,9 @@ struct DensLane {
         if (PER && per) min_image(*gp, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R rr = f_sqrt(r2);
-        const bool in = live && ((sizeof(R) == 4) ? (r2 < th2) : (rr < two_h));
+        // `&`, not `&&`: a short-circuit lets the compiler branch around (and sink
+        // the record load into) every slot's test
+        const bool in = live & ((sizeof(R) == 4) ? (r2 < th2) : (rr < two_h));
         // the mass multiplies unconditionally, so the record stays one 16-byte load
         return q.w * (in ? sig * m4_shape(rr * inv_h) : R(0));
     }
@@ -451,15 +453,15 @@ __device__ R density_walk(const Carried<R>& c, const Work<R>& w, const SegPar& p
         L.frame(sx, sy, sz1, bx, by, bz1);
         if (!L.emu) {
             // four loads in flight, also for a piece's last partial group: its
-            // spare slots re-read the piece's last record and add zero
-            const uint32_t last = k1 - 1;
+            // spare slots read the records that follow (the record arrays hold
+            // kRecordPad zeroed records past the last particle) and add zero
             for (uint32_t k = k0; k < k1; k += 4) {
-                const uint32_t ka = min(k + 1, last), kb = min(k + 2, last), kc = min(k + 3, last);
-                const V4 q0 = w.dens_rec[k], q1 = w.dens_rec[ka], q2 = w.dens_rec[kb], q3 = w.dens_rec[kc];
+                const V4* rec = w.dens_rec + k;
+                const V4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
                 s += L.term(q0, bx, by, k < ks ? bz0 : bz1);
-                s += L.term(q1, bx, by, ka < ks ? bz0 : bz1, k + 1 < k1);
-                s += L.term(q2, bx, by, kb < ks ? bz0 : bz1, k + 2 < k1);
-                s += L.term(q3, bx, by, kc < ks ? bz0 : bz1, k + 3 < k1);
+                s += L.term(q1, bx, by, k + 1 < ks ? bz0 : bz1, k + 1 < k1);
+                s += L.term(q2, bx, by, k + 2 < ks ? bz0 : bz1, k + 2 < k1);
+                s += L.term(q3, bx, by, k + 3 < ks ? bz0 : bz1, k + 3 < k1);
             }
         } else {
             for (uint32_t k = k0; k < k1; k++) {
@@ -519,34 +521,42 @@ __device__ double density_walk_hp(const Carried<float>& c, const Work<float>& w,
         L.frame(sx, sy, sz0, bx, by, bz0);
         L.frame(sx, sy, sz1, bx, by, bz1);
         if (!L.emu) {
-            // in chunks of up to 64 candidates: the screen (four records in flight)
+            // In chunks of up to 64 candidates: the screen (four records in flight;
+            // the record arrays hold kRecordPad records past the last particle, so a
+            // group's loads never leave them, and slots past the chunk never count)
             // sets a bit per candidate that may be in range, then the FP64 terms run
-            // over the set bits two at a time, in increasing k
+            // over the set bits two at a time, in increasing k, both records loaded
+            // before either term (the second term adds exactly zero when the chunk
+            // has one bit left, and the sum is never -0).
             const float th2 = L.th2 * 1.00002f;
             for (uint32_t c0 = k0; c0 < k1; c0 += 64u) {
-                const uint32_t c1 = min(c0 + 64u, k1), last = c1 - 1;
+                const uint32_t c1 = min(c0 + 64u, k1);
                 uint64_t mask = 0;
                 for (uint32_t k = c0; k < c1; k += 4) {
+                    const float4* rec = w.dens_rec + k;
                     float4 q[4];
 #pragma unroll
-                    for (int b = 0; b < 4; b++) q[b] = w.dens_rec[min(k + b, last)];
+                    for (int b = 0; b < 4; b++) q[b] = rec[b];
+                    uint32_t nib = 0;
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
                         float rx = bx - q[b].x, ry = by - q[b].y, rz = (k + b < ks ? bz0 : bz1) - q[b].z;
                         if (PER && L.per) min_image(g, rx, ry, rz);
-                        const bool in = k + b < c1 && rx * rx + ry * ry + rz * rz < th2;
-                        mask |= (uint64_t)in << (k + b - c0);
+                        nib |= (uint32_t)((k + b < c1) & (rx * rx + ry * ry + rz * rz < th2)) << b;
                     }
+                    mask |= (uint64_t)nib << (k - c0);
                 }
+                const double4* rec = w.dens_rec64 + c0;
                 while (mask) {
-                    const uint32_t ka = c0 + (uint32_t)__builtin_ctzll(mask);
+                    const uint32_t ba = (uint32_t)__builtin_ctzll(mask);
                     mask &= mask - 1;
                     const bool two = mask != 0;
-                    const uint32_t kb = two ? c0 + (uint32_t)__builtin_ctzll(mask) : ka;
-                    if (two) mask &= mask - 1;
-                    const double4 qa = w.dens_rec64[ka], qb = w.dens_rec64[kb];
-                    s += term(qa);
-                    if (two) s += term(qb);
+                    const uint32_t bb = two ? (uint32_t)__builtin_ctzll(mask) : ba;
+                    mask &= mask - 1;
+                    const double4 qa = rec[ba], qb = rec[bb];
+                    const double ta = term(qa), tb = term(qb);
+                    s += ta;
+                    s += two ? tb : 0.0;
                 }
             }
         } else {
@@ -911,6 +921,7 @@ __global__ __launch_bounds__(256) void k_eos_records(Carried<R> c, Work<R> w) {
 
 // ---------------------------------------------------------------- forces
 constexpr int kForceBatch = 4;   // position records in flight per lane (force cell walk)
+static_assert(kForceBatch <= kRecordPad + 1, "a screen group reads at most kRecordPad records past a run");
 // Fused strain rate + forces (sph.hpp:212-258 and 277-317; the Metal
 // strain_forces kernel, sph_force.metal:308-365, fuses the same two loops).
 // For each pair j != i with 0 < r < 2 hbar, hbar = (h_i + h_j)/2:
@@ -1026,7 +1037,7 @@ struct ForceLane {
         if (PER && per) min_image(*gp, rx, ry, rz);
         r2 = rx * rx + ry * ry + rz * rz;
         th = hi + q.w;
-        return live && r2 < th * th && r2 != R(0);
+        return live & (r2 < th * th) & (r2 != R(0));   // no short-circuit: one basic block per slot
     }
 
     // strain rate, spin, Jaumann rate and deviatoric power (sph.hpp:230-256),
@@ -1116,25 +1127,29 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                         bx = F.pi.x; by = F.pi.y; bz0 = bz1 = F.pi.z;
                     }
                     // In chunks of up to 64 candidates: screen the position records
-                    // (kForceBatch in flight; a chunk's last partial group re-reads its
-                    // last record, never in range) into a bit mask, then run the pair
-                    // body over the set bits only, in increasing k as before.  A wave's
+                    // (kForceBatch in flight; the record arrays hold kRecordPad records
+                    // past the last particle, so a group's loads never leave them, and
+                    // slots past the chunk are never in range) into a bit mask, then run
+                    // the pair body over the set bits only, in increasing k.  A wave's
                     // body loop then takes as many turns as its lane with the most pairs
                     // in range, not as its lane with the longest run.
                     for (uint32_t c0 = k0; c0 < k1; c0 += 64u) {
-                        const uint32_t c1 = min(c0 + 64u, k1), last = c1 - 1;
+                        const uint32_t c1 = min(c0 + 64u, k1);
                         uint64_t mask = 0;
                         for (uint32_t k = c0; k < c1; k += kForceBatch) {
+                            const typename Vec4<R>::T* rec = w.frec_pos + k;
                             typename Vec4<R>::T q[kForceBatch];
 #pragma unroll
-                            for (int b = 0; b < kForceBatch; b++) q[b] = w.frec_pos[min(k + b, last)];
+                            for (int b = 0; b < kForceBatch; b++) q[b] = rec[b];
+                            uint32_t nib = 0;
 #pragma unroll
                             for (int b = 0; b < kForceBatch; b++) {
                                 R rx, ry, rz, r2, th;
                                 const bool in = F.in_range(q[b], bx, by, k + b < ks ? bz0 : bz1, rx, ry, rz, r2, th,
                                                            k + b < c1);
-                                mask |= (uint64_t)in << (k + b - c0);
+                                nib |= (uint32_t)in << b;
                             }
+                            mask |= (uint64_t)nib << (k - c0);
                         }
                         while (mask) {
                             const uint32_t k = c0 + (uint32_t)__builtin_ctzll(mask);

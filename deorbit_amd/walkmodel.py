@@ -17,6 +17,8 @@ candidates visited and the pairs in range, and per wave the turns of
     wave's longest run in each column;
   * ``mask``: the run's pairs in range only (the bit-mask walk of the kernels):
     turns = the wave's largest count of pairs in range per column;
+  * ``flat``: each lane's pairs in range queued across all its columns and run
+    one per turn: turns = the wave's largest count of pairs in range per lane;
   * ``ideal``: every lane busy: the wave's pairs in range / 64.
 """
 from __future__ import annotations
@@ -67,10 +69,11 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
     hmax = H.max()
     rng = np.random.default_rng(seed)
     waves = rng.choice(len(P) // 64, min(nwaves, len(P) // 64), replace=False)
-    tot = dict(cand=0, inr=0, slot=0, mask=0, ideal=0)
+    tot = dict(cand=0, inr=0, slot=0, mask=0, flat=0, ideal=0)
     for wv in waves:
         cols = {}
         inr_wave = 0
+        lane_inr = np.zeros(64, int)
         for lane, i in enumerate(range(wv * 64, wv * 64 + 64)):
             hi = H[i]
             reach = 2 * hi if which == "density" else hi + hmax
@@ -118,11 +121,13 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
                         nin = int(((r < hi + H[k0:k1]) & (r > 0)).sum())
                     cols.setdefault((dx, dy), np.zeros((64, 2), int))[lane] = (k1 - k0, nin)
                     inr_wave += nin
+                    lane_inr[lane] += nin
                     tot["cand"] += k1 - k0
                     tot["inr"] += nin
         A = np.array(list(cols.values()))
         tot["slot"] += float((np.ceil(A[:, :, 0].max(1) / 4) * 4).sum()) / 4
         tot["mask"] += float(A[:, :, 1].max(1).sum())
+        tot["flat"] += int(lane_inr.max())
         tot["ideal"] += math.ceil(inr_wave / 64)
     n = len(waves) * 64
     return {"particles": len(P), "waves": len(waves), "cell_factor": cell_factor, "pass": which,
@@ -130,6 +135,7 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
             "candidates_per_particle": tot["cand"] / n, "pairs_per_particle": tot["inr"] / n,
             "candidates_per_pair": tot["cand"] / max(1, tot["inr"]),
             "slot_turns_per_wave": tot["slot"] / len(waves), "mask_turns_per_wave": tot["mask"] / len(waves),
+            "flat_turns_per_wave": tot["flat"] / len(waves),
             "ideal_turns_per_wave": tot["ideal"] / len(waves),
             "mask_lane_use": tot["inr"] / 64 / max(1.0, tot["mask"])}
 
