@@ -339,9 +339,10 @@ constexpr float kSliceMargin = 1e-3f;
 // within reach along z (at most two slice blocks within the tuning limits,
 // hence one frame split).  A wave's
 // lanes sit in one column and walk the same column sequence with runs of
-// similar length, and the z range is cut to the lane's own reach rather than
-// to whole cubic cells.  Periodic segments walk cell by cell.
-template <class R, bool PER, bool HB, class RF, class F>
+// similar length (MIR mirrors the sequence for half of them, below), and the
+// z range is cut to the lane's own reach rather than to whole cubic cells.
+// Periodic segments walk cell by cell.
+template <class R, bool PER, bool HB, bool MIR = false, class RF, class F>
 __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, int cx, int cy, int cz, R ox, R oy,
                                            R ozs, R reach_max, RF&& reach_of, F&& fn) {
     const R rmax2 = reach_max * reach_max;
@@ -351,13 +352,21 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
         rx = rx < 1 ? 1 : (rx > kMaxStencil ? kMaxStencil : rx);
         ry = ry < 1 ? 1 : (ry > kMaxStencil ? kMaxStencil : ry);
         const int fi = block_start(cz), zmax = g.dim[2] - 1;
-        for (int dx = -rx; dx <= rx; dx++) {
+        // With MIR, lanes in the upper half of their column along x (y) take the
+        // columns in mirrored order along x (y): at each step of the walk the
+        // wave's lanes then sit in columns where they have similar numbers of
+        // pairs in range (in at most four columns), which evens out the turns of
+        // a pair body that runs over the pairs in range only.
+        const int mx = (MIR && ox >= R(0.5) * wx) ? -1 : 1, my = (MIR && oy >= R(0.5) * wy) ? -1 : 1;
+        for (int ex = -rx; ex <= rx; ex++) {
+            const int dx = mx * ex;
             const int nx = cx + dx;
             if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
             const R gx = cell_gap(dx, ox, wx);
             const R gx2 = gx * gx;
             if (gx2 >= rmax2) continue;
-            for (int dy = -ry; dy <= ry; dy++) {
+            for (int ey = -ry; ey <= ry; ey++) {
+                const int dy = my * ey;
                 const int ny = cy + dy;
                 if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
                 const R gy = cell_gap(dy, oy, wy);
@@ -1094,8 +1103,9 @@ __device__ __forceinline__ bool force_list_usable(const Work<R>& w, int64_t i, i
 }
 
 // Force pass by walking the columns (radius h_i + max h, culled with the
-// per-block bound of h).  With lists built, particles whose list is usable are left to
-// k_force_list.
+// per-block bound of h), in the mirrored column order (for_pieces, MIR).  With
+// lists built, particles whose list is usable are left to k_force_list, which
+// sums in the list's order (the density walk's).
 template <class R, bool PER>
 __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, int lists, SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -1116,8 +1126,8 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
             own_offset<R>(c, g, i, cc, w.dens_rec[i], F.ox, F.oy, F.oz);
             F.ozs = slice_offset<R>(g, F.cz, F.oz);
             if (isfinite((double)F.ox) && isfinite((double)F.oy) && isfinite((double)F.oz)) {
-                for_pieces<R, PER, true>(g, w, F.cx, F.cy, F.cz, F.ox, F.oy, F.ozs,
-                                         F.hi + (R)w.st[seg].hmax_final, [&](R hb) { return F.hi + hb; },
+                for_pieces<R, PER, true, true>(g, w, F.cx, F.cy, F.cz, F.ox, F.oy, F.ozs,
+                                               F.hi + (R)w.st[seg].hmax_final, [&](R hb) { return F.hi + hb; },
                                          [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1,
                                              R, const PieceFrame&) {
                     R bx, by, bz0, bz1;

@@ -54,12 +54,13 @@ def test_graph_replay_equals_stream_launches(precision, gpu):
 
 @pytest.mark.parametrize("precision", [dsph.FP64, dsph.MIXED_FAST])
 def test_neighbour_lists_give_the_cell_walk_results(precision, gpu):
-    # The lists hold the walk's pairs in the walk's order, so every way of
+    # The lists hold the density walk's pairs in its order, so every way of
     # falling back (lists off, lists overflowing, h outgrowing a tiny skin)
     # must reproduce the same numbers up to the compiler contracting
-    # multiply-adds differently in the list and walk kernels: round-off of the
-    # working precision (scale-normalised 1e-13 in FP64, 1e-5 in FP32 after 12
-    # steps).
+    # multiply-adds differently in the list and walk kernels and the force walk
+    # taking the columns in mirrored order for half of the lanes: round-off of
+    # the working precision (scale-normalised 1e-13 in FP64, 1e-5 in FP32 after
+    # 12 steps).
     ic = ics.impact_37k()
     steps = 12
     ref, dref = run_states(ic, precision, steps, True, dict(neighbor_capacity=-1))
