@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -36,6 +38,11 @@ static thread_local std::string g_last_error;
         if (e_ != hipSuccess)                                                                     \
             throw Err{DSPH_E_HIP, std::string(#x) + " failed: " + hipGetErrorString(e_)};        \
     } while (0)
+
+// Largest cell table the engine grows to on its own, in cells per particle
+// (the bound set_tuning accepts too).
+constexpr int kMaxCellsPerParticle = 64;
+constexpr int kDefaultCellsPerParticle = 4;
 
 static void check_launch(const char* what) {
     hipError_t e = hipGetLastError();
@@ -84,7 +91,7 @@ class Engine : public EngineBase {
         tune_.block_point = 256;
         tune_.cell_factor = 1.5;
         tune_.xcd_remap = 1;
-        tune_.cell_capacity_per_particle = 4;
+        tune_.cell_capacity_per_particle = kDefaultCellsPerParticle;
         tune_.neighbor_capacity = 0;
         tune_.list_skin = 0.06;
         for (const auto& p : segp_)
@@ -165,7 +172,8 @@ class Engine : public EngineBase {
         for (int k = 0; k < 2; k++) { keys_[k] = dalloc<uint32_t>(n_); vals_[k] = dalloc<uint32_t>(n_); }
         hist_ = dalloc<uint32_t>((size_t)256 * nb_sort_);
         digit_totals_ = dalloc<uint32_t>(256);
-        d_total_cells_ = dalloc<uint32_t>(1);
+        d_total_cells_ = dalloc<uint32_t>(2);
+        zero(d_total_cells_, 2);
         // per-segment tables
         std::vector<MatT<R>> hm(mats_.size());
         for (size_t k = 0; k < mats_.size(); k++) hm[k] = to_mat(mats_[k]);
@@ -671,7 +679,58 @@ class Engine : public EngineBase {
             const int rc = seg_status(st[s], s, &msg);
             if (rc != DSPH_OK) throw Err{rc, msg};
         }
+        grow_table();
         return 0;
+    }
+
+    // Cell table growth.  The sort grid spans the particles' bounding box, which
+    // a few fast particles can stretch far beyond the bulk; when the nominal
+    // widths need more cells than the table holds, k_grid widens every cell
+    // (correct, but every walk then visits more candidates).  After a call has
+    // synchronised, the table is regrown to 1.25x what the last grid wanted, up
+    // to kMaxCellsPerParticle cells per particle, so that later steps run at the
+    // tuned widths (the autotuner grows it before timing a candidate and sizes
+    // it for the chosen one).  The cells' order, hence each pair sum's order,
+    // changes with the widths: a context's results depend on when its table
+    // grew at round-off level only.  Returns true when the table was
+    // reallocated.
+    uint32_t wanted_cells() {
+        uint32_t tc[2] = {0, 0};
+        HIPC(hipMemcpyAsync(tc, d_total_cells_, sizeof(tc), hipMemcpyDeviceToHost, stream_));
+        sync();
+        return tc[1];
+    }
+    bool grow_table() {
+        if (n_ <= 0 || !d_total_cells_) return false;
+        const uint32_t wanted = wanted_cells();
+        if (wanted <= capacity_) return false;
+        return resize_table(wanted, false);
+    }
+    // A table of 1.25x `wanted` cells (at least the default 4 per particle, at
+    // most kMaxCellsPerParticle); only ever smaller than the current one if
+    // `shrink`.
+    bool resize_table(uint32_t wanted, bool shrink) {
+        const int cur = tune_.cell_capacity_per_particle;
+        const double want = 1.25 * (double)wanted / (double)n_;
+        const int per = (int)std::max<double>(kDefaultCellsPerParticle,
+                                              std::min<double>(kMaxCellsPerParticle, std::ceil(want)));
+        if (per == cur || (per < cur && !shrink)) return false;
+        release_graphs();
+        for (void* old : {(void*)w_.cstart, (void*)w_.cell_hmax}) free_ptr(old);
+        tune_.cell_capacity_per_particle = per;
+        try {
+            alloc_table();
+        } catch (const Err&) {
+            // not enough memory: back to a table of the old size (coarser cells)
+            (void)hipGetLastError();
+            free_ptr(w_.cstart);
+            tune_.cell_capacity_per_particle = cur;
+            alloc_table();
+            return false;
+        }
+        built_ = false;
+        records_ = false;
+        return true;
     }
 
     void timestep(double* out) {
@@ -893,7 +952,7 @@ class Engine : public EngineBase {
         }
         const bool lists_changed = nt.neighbor_capacity != tune_.neighbor_capacity || nt.list_skin != tune_.list_skin;
         if (t.cell_capacity_per_particle && t.cell_capacity_per_particle != tune_.cell_capacity_per_particle) {
-            if (t.cell_capacity_per_particle < 1 || t.cell_capacity_per_particle > 64)
+            if (t.cell_capacity_per_particle < 1 || t.cell_capacity_per_particle > kMaxCellsPerParticle)
                 throw Err{DSPH_E_ARG, "cell_capacity_per_particle must be in [1, 64]"};
             nt.cell_capacity_per_particle = t.cell_capacity_per_particle;
             if (n_ > 0) {
@@ -950,9 +1009,12 @@ class Engine : public EngineBase {
         }
         const int saved_cur = cur_;
         const bool saved_prof = profiling_;
+        const char* log_env = std::getenv("DSPH_AUTOTUNE_LOG");
+        const bool log = log_env && *log_env && std::strcmp(log_env, "0") != 0;   // each candidate's time to stderr
         profiling_ = false;
         dsph_tuning best = tune_;
         double best_ms = 1e300;
+        uint32_t best_wanted = 0;       // cells the fastest candidate's grid wanted
         const int blocks[3] = {64, 128, 256};
         // column widths in h_init; on lattice initial conditions the step time
         // varies sharply around 1.5 (how lattice lines fall into columns), hence
@@ -990,25 +1052,38 @@ class Engine : public EngineBase {
                 tune_ = t;
                 restore();
                 step(0, 0.0);           // warm-up (also rebuilds for this grid)
+                sync();
+                if (grow_table()) {     // time the candidate at its nominal widths
+                    restore();
+                    step(0, 0.0);
+                }
                 HIPC(hipEventRecord(e0, stream_));
                 for (int s = 0; s < steps; s++) step(0, 0.0);
                 HIPC(hipEventRecord(e1, stream_));
                 HIPC(hipEventSynchronize(e1));
                 float ms = 0.f;
                 HIPC(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best_ms) { best_ms = ms; best = t; }
+                const uint32_t wanted = wanted_cells();
+                if (log)
+                    std::fprintf(stderr, "dsph autotune: block_pair %d cell_factor %.2f neighbor_capacity %d: %.3f ms/step\n",
+                                 bq, cf, cap, ms / steps);
+                if (ms < best_ms) { best_ms = ms; best = t; best_wanted = wanted; }
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         release_graphs();
         set_lists(best.neighbor_capacity);
+        best.cell_capacity_per_particle = tune_.cell_capacity_per_particle;   // the table as grown
         tune_ = best;
+        // size the table for the chosen widths (the narrowest candidates grew it
+        // further; its size sets the number of radix passes)
+        if (best_wanted) resize_table(best_wanted, true);
         restore();
         sync();
         for (void* q : saved) (void)hipFree(q);
         profiling_ = saved_prof;
-        if (chosen) *chosen = best;
+        if (chosen) *chosen = tune_;   // best, with the table as sized above
     }
 
     // host-side Weibull seeding (sph.hpp:117-128) over one segment in IC order

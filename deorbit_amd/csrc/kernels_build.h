@@ -98,6 +98,9 @@ __global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int
 // and y cut into slices of w * kSliceRatio along z, over the segment bbox (at
 // most 1001 cells per axis), segments laid end to end in one key space.  If the total exceeds the table capacity every width grows by 25 %
 // until it fits (coarser cells stay correct: stencils are sized in cells).
+// total_out[0] is the total laid out, total_out[1] the total the nominal
+// widths would need (saturated at 2^32 - 1), from which the host grows the
+// table between calls.
 // One workgroup of kGridThreads threads: segments are handled a chunk at a
 // time, one per thread, and a block scan lays their key ranges end to end.
 constexpr int kGridThreads = 1024;
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
     __shared__ unsigned long long s_carry;
     const int t = threadIdx.x;
     double scale = 1.0;
-    unsigned long long total = 0;
+    unsigned long long total = 0, wanted = 0;
     for (int attempt = 0; attempt < 200; attempt++) {
         if (t == 0) s_carry = 0;
         __syncthreads();
@@ -178,12 +181,14 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
         }
         total = s_carry;
         __syncthreads();
+        if (attempt == 0) wanted = total;
         if (total <= capacity) break;
         scale *= 1.25;
     }
     if (t == 0) {
         if (total > capacity) { st[0].bad |= 1; total = capacity; }
-        *total_out = (uint32_t)total;
+        total_out[0] = (uint32_t)total;
+        total_out[1] = wanted > 0xffffffffull ? 0xffffffffu : (uint32_t)wanted;
     }
 }
 

@@ -87,7 +87,10 @@ typedef struct {
     int32_t block_point;        /* threads per workgroup, per-particle kernels   */
     double cell_factor;         /* sort-column width = cell_factor * h_init (z slices: a quarter of it) */
     int32_t xcd_remap;          /* 1 = map consecutive tiles onto one XCD        */
-    int32_t cell_capacity_per_particle;  /* cell table size / particle count    */
+    int32_t cell_capacity_per_particle;  /* cell table size / particle count; the
+                                 * library grows it (up to 64) after a call whose
+                                 * sort grid wanted more cells than the table held
+                                 * (DSPH_AUTOTUNE_LOG=1 prints autotune candidates) */
     int32_t neighbor_capacity;  /* neighbour-list entries per particle; 0 = auto, <0 = no lists */
     double list_skin;           /* list radius = (1 + skin) x interaction radius */
 } dsph_tuning;
