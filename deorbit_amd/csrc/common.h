@@ -80,6 +80,9 @@ struct SegState {
     uint32_t pad_;
     unsigned long long list_walk_steps;    // force passes that walked cells for that reason
     unsigned long long list_overflow;      // particle lists that did not fit (cumulative)
+    // density moment walks (counted only with Work::stats): lanes and waves of
+    // density rounds, and those whose second iteration walked again
+    unsigned long long dens_lanes, dens_rewalks, dens_waves, dens_rewalk_waves;
 };
 
 // Device-side carried particle state, one buffer set.
@@ -140,6 +143,7 @@ struct Work {
     SegState* st;
     int64_t n;
     int32_t nseg;
+    int32_t stats;                   // non-zero: count the density moment walks (SegState::dens_*)
 };
 
 // ---- small device helpers ---------------------------------------------------

@@ -147,6 +147,10 @@ class Engine : public EngineBase {
         Work<R>& w = w_;
         w.n = n_;
         w.nseg = nseg_;
+        {   // DSPH_COUNT_REWALKS=1: count the density moment walks (dsph_diag.density_*)
+            const char* e = std::getenv("DSPH_COUNT_REWALKS");
+            w.stats = (e && e[0] == '1') ? 1 : 0;
+        }
         w.ax = dalloc<R>(n_); w.ay = dalloc<R>(n_); w.az = dalloc<R>(n_);
         w.rho = dalloc<R>(n_); w.P = dalloc<R>(n_); w.cs = dalloc<R>(n_); w.csig = dalloc<R>(n_);
         w.Pf = dalloc<R>(n_); w.Rart = dalloc<R>(n_); w.dscale = dalloc<R>(n_); w.sigmax = dalloc<R>(n_);
@@ -357,6 +361,11 @@ class Engine : public EngineBase {
     }
     template <class K, class... Args>
     void launch(int phase, const char* name, K kernel, dim3 grid, dim3 block, Args... args) {
+        launch_lds(phase, name, kernel, grid, block, 0, args...);
+    }
+    // launch with `lds` bytes of dynamic LDS per workgroup
+    template <class K, class... Args>
+    void launch_lds(int phase, const char* name, K kernel, dim3 grid, dim3 block, size_t lds, Args... args) {
         hipEvent_t ea = nullptr, eb = nullptr;
         const bool prof = profiling_ && !capturing_;
         if (prof) {
@@ -364,7 +373,7 @@ class Engine : public EngineBase {
             eb = get_event();
             HIPC(hipEventRecord(ea, stream_));
         }
-        hipLaunchKernelGGL(kernel, grid, block, 0, stream_, args...);
+        hipLaunchKernelGGL(kernel, grid, block, lds, stream_, args...);
         check_launch(name);
         if (prof) {
             HIPC(hipEventRecord(eb, stream_));
@@ -484,8 +493,8 @@ class Engine : public EngineBase {
             const auto th = hmax_target(r + 1);
             if constexpr (sizeof(R) == 4) {
                 if (density64_) {
-                    launch(PH_DENSITY, "density", k_density_round_hp<PER>, dim3(nb), dim3(bq), cur(),
-                           w_, r, tune_.xcd_remap, th);
+                    launch_lds(PH_DENSITY, "density", k_density_round_hp<PER>, dim3(nb), dim3(bq),
+                               dens_band_lds(bq), cur(), w_, r, tune_.xcd_remap, th);
                     fold_parts(PH_DENSITY, th, nb);
                     continue;
                 }
@@ -496,8 +505,8 @@ class Engine : public EngineBase {
                 launch(PH_DENSITY, "density", k_density<R, true, PER>, dim3(nb), dim3(bq), cur(), w_, r,
                        tune_.xcd_remap, lists, th);
             } else {
-                launch(PH_DENSITY, "density", k_density_round<R, PER>, dim3(nb), dim3(bq), cur(), w_, r,
-                       tune_.xcd_remap, lists, th);
+                launch_lds(PH_DENSITY, "density", k_density_round<R, PER>, dim3(nb), dim3(bq),
+                           sizeof(R) == 8 ? dens_band_lds(bq) : 0, cur(), w_, r, tune_.xcd_remap, lists, th);
             }
             fold_parts(PH_DENSITY, th, nb);
             if (r == 0 && lists) {
@@ -904,6 +913,10 @@ class Engine : public EngineBase {
         out->density_rounds = s.rounds;
         out->list_walk_steps = (int64_t)s.list_walk_steps;
         out->list_overflow = (int64_t)s.list_overflow;
+        out->density_lanes = (int64_t)s.dens_lanes;
+        out->density_rewalks = (int64_t)s.dens_rewalks;
+        out->density_waves = (int64_t)s.dens_waves;
+        out->density_rewalk_waves = (int64_t)s.dens_rewalk_waves;
         out->status = seg_status(s, seg, nullptr);
     }
 

@@ -578,6 +578,223 @@ __device__ double density_walk_hp(const Carried<float>& c, const Work<float>& w,
     return s;
 }
 
+// ---------------------------------------------------------------- moment walks
+// A walk that lets the second iteration of a density round (sph.hpp:189-198)
+// skip its own walk.  With q = r/h the M4 shape is 1 - 1.5 q^2 + 0.75 q^3 for
+// q < 1 and (2 - q)^3 / 4 for 1 <= q < 2, so for every h1 with
+// |h1/h - 1| <= d/2
+//   sum_j m_j f(r_j / h1) = A0 - 1.5 A2 / h1^2 + 0.75 A3 / h1^3
+//                           + (B3 + 3 D B2 + 3 D^2 B1 + D^3 B0) / (4 h1^3)
+//                           + sum over the band candidates of m_j f(r_j / h1)
+// with An = sum m r^n over q < 1 - d, Bn = sum m u^n with u = 2h - r over
+// 1 + d <= q < 2 - 2d, and D = 2 (h1 - h): these candidates stay on the same
+// side of h1 as of h and within 2 h1 (u + D = 2 h1 - r > 0).  The band
+// candidates (1 - d <= q < 1 + d and 2 - 2d <= q < 2 + 2d) are kept by index
+// and summed directly at h1 with the reference's own test r < 2 h1; about
+// 3 N (2d) (2^3 + 1) / 2^3 of a particle's N neighbours fall in them.  The B
+// sum has no cancellation (|D| < u); the A sum cancels at most fourfold, as
+// the shape itself does at q = 1.  A particle whose h1 leaves the band, whose
+// bands held more than kDensBand candidates, or whose 2 h1 may exceed the
+// reference's cell width (emulation) walks again.
+constexpr int kDensBand = 8;
+constexpr double kDensBandRel = 5e-3;
+
+struct DensMom {
+    double a0, a2, a3, b0, b1, b2, b3;
+    double sb;      // the band candidates' terms at the walk's h
+    int nb;         // band candidates met (the first kDensBand are kept)
+    __device__ __forceinline__ void clear() {
+        a0 = a2 = a3 = b0 = b1 = b2 = b3 = sb = 0.0;
+        nb = 0;
+    }
+    // The moments' share of a candidate of mass m at distance r (r2 as formed,
+    // q = r/h, two_h = 2h); true for a band candidate, which it leaves out.
+    __device__ __forceinline__ bool add(double m, double r, double r2, double q, double two_h) {
+        const bool inner = q < 1.0 - kDensBandRel;
+        const bool outer = (q >= 1.0 + kDensBandRel) & (q < 2.0 - 2.0 * kDensBandRel);
+        const double a = inner ? m : 0.0, ar2 = a * r2;
+        a0 += a;
+        a2 += ar2;
+        a3 = fma(ar2, r, a3);
+        const double b = outer ? m : 0.0, u = two_h - r, bu = b * u, bu2 = bu * u;
+        b0 += b;
+        b1 += bu;
+        b2 += bu2;
+        b3 = fma(bu2, u, b3);
+        return !(inner | outer) & (q < 2.0 + 2.0 * kDensBandRel);
+    }
+    // sum_j m_j f(r_j / h1) over the moments of a walk at h (band excluded)
+    __device__ __forceinline__ double at(double h, double h1) const {
+        const double ih = 1.0 / h1, ih2 = ih * ih, ih3 = ih2 * ih;
+        const double D = 2.0 * (h1 - h);
+        const double bs = fma(D, fma(D, fma(D, b0, 3.0 * b1), 3.0 * b2), b3);
+        return a0 - 1.5 * a2 * ih2 + 0.75 * a3 * ih3 + 0.25 * bs * ih3;
+    }
+    // may the iteration at h1 use the moments of a walk at h?
+    __device__ __forceinline__ bool covers(double h, double h1) const {
+        return nb <= kDensBand && h1 >= h * (1.0 - 0.5 * kDensBandRel) && h1 <= h * (1.0 + 0.5 * kDensBandRel);
+    }
+};
+
+// Counters of the moment walks, kept only when Work::stats is set: lanes of
+// density rounds, lanes whose second iteration walked again, and waves with
+// such a lane, added per wave to the segment of the wave's first lane.  Every
+// lane of the wave must call this.
+template <class R>
+__device__ __forceinline__ void count_rewalks(const Work<R>& w, int seg, bool act, bool rewalk) {
+    if (!w.stats) return;
+    const unsigned long long na = __ballot(act), nr = __ballot(rewalk);
+    const int s0 = __shfl(seg, 0);
+    if (lane_id() == 0) {
+        SegState& S = w.st[s0];
+        atomicAdd(&S.dens_lanes, (unsigned long long)__popcll(na));
+        if (nr) {
+            atomicAdd(&S.dens_rewalks, (unsigned long long)__popcll(nr));
+            atomicAdd(&S.dens_rewalk_waves, 1ull);
+        }
+        if (na) atomicAdd(&S.dens_waves, 1ull);
+    }
+}
+
+// Lane slots of the band candidates: slot s of the lane at threadIdx.x t sits
+// at [s * blockDim.x + t] of the block's dynamic LDS.
+__device__ __forceinline__ uint32_t* dens_band_slots() {
+    extern __shared__ uint32_t dsph_dyn_lds[];
+    return dsph_dyn_lds + threadIdx.x;
+}
+constexpr size_t dens_band_lds(int block) { return (size_t)kDensBand * (size_t)block * sizeof(uint32_t); }
+
+// density_walk_hp with the moments of the walk (reach and screen widened to
+// 2h (1 + d) for the outer band); fills M and the band slots.  Lanes that
+// emulate the reference's cells do not come here.
+template <bool PER>
+__device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, const DensLane<float, PER>& L,
+                                    const DensHP& H, DensMom& M, uint32_t* band) {
+    M.clear();
+    const uint32_t bstride = blockDim.x;
+    auto add = [&](const double4& q, uint32_t k, bool live) {
+        double rx = H.x - q.x, ry = H.y - q.y, rz = H.z - q.z;
+        if (PER && L.per) min_image(g, rx, ry, rz);
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double r = sqrt_seeded(r2);
+        const double qq = r * H.inv_h;
+        if (M.add(live ? q.w : 0.0, r, r2, qq, H.two_h) & live) {
+            if (M.nb < kDensBand) band[M.nb * bstride] = k;
+            M.nb++;
+            M.sb += r < H.two_h ? q.w * m4_shape(qq) : 0.0;
+        }
+    };
+    const float grow = (float)(1.0 + kDensBandRel);
+    for_pieces<float, PER, false>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h * grow * 1.00001f,
+                                  [](float) { return 0.0f; },
+                                  [&](uint32_t k0, uint32_t k1, uint32_t ks, float sx, float sy, float sz0, float sz1,
+                                      float, const PieceFrame&) {
+        float bx, by, bz0, bz1;
+        L.frame(sx, sy, sz0, bx, by, bz0);
+        L.frame(sx, sy, sz1, bx, by, bz1);
+        const float th2 = L.th2 * (grow * grow * 1.00002f);
+        for (uint32_t c0 = k0; c0 < k1; c0 += 64u) {
+            const uint32_t c1 = min(c0 + 64u, k1);
+            uint64_t mask = 0;
+            for (uint32_t k = c0; k < c1; k += 4) {
+                const float4* rec = w.dens_rec + k;
+                float4 q[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) q[b] = rec[b];
+                uint32_t nib = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    float rx = bx - q[b].x, ry = by - q[b].y, rz = (k + b < ks ? bz0 : bz1) - q[b].z;
+                    if (PER && L.per) min_image(g, rx, ry, rz);
+                    nib |= (uint32_t)((k + b < c1) & (rx * rx + ry * ry + rz * rz < th2)) << b;
+                }
+                mask |= (uint64_t)nib << (k - c0);
+            }
+            const double4* rec = w.dens_rec64 + c0;
+            while (mask) {
+                const uint32_t ba = (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                const bool two = mask != 0;
+                const uint32_t bb = two ? (uint32_t)__builtin_ctzll(mask) : ba;
+                mask &= mask - 1;
+                const double4 qa = rec[ba], qb = rec[bb];
+                add(qa, c0 + ba, true);
+                add(qb, c0 + bb, two);
+            }
+        }
+    });
+}
+
+// The band candidates' terms at the h of H (FP64 records, as the walk formed r).
+template <bool PER>
+__device__ double dens_band_hp(const Work<float>& w, const SegGrid& g, const DensLane<float, PER>& L,
+                               const DensHP& H, const DensMom& M, const uint32_t* band) {
+    double s = 0.0;
+    const uint32_t bstride = blockDim.x;
+    for (int b = 0; b < M.nb; b++) {
+        const double4 q = w.dens_rec64[band[b * bstride]];
+        double rx = H.x - q.x, ry = H.y - q.y, rz = H.z - q.z;
+        if (PER && L.per) min_image(g, rx, ry, rz);
+        const double r = sqrt_seeded(rx * rx + ry * ry + rz * rz);
+        s += r < H.two_h ? q.w * m4_shape(r * H.inv_h) : 0.0;
+    }
+    return s;
+}
+
+// The FP64 path's walk (density_walk<double>, every slot, four records in
+// flight) with the moments of the walk, reach widened to 2h (1 + d); fills M
+// and the band slots.  Lanes that emulate the reference's cells do not come
+// here.
+template <bool PER>
+__device__ void density_walk_mom(const Work<double>& w, const SegGrid& g, const DensLane<double, PER>& L,
+                                 DensMom& M, uint32_t* band) {
+    M.clear();
+    const uint32_t bstride = blockDim.x;
+    auto add = [&](const double4& q, uint32_t k, bool live) {
+        double rx = L.me.x - q.x, ry = L.me.y - q.y, rz = L.me.z - q.z;
+        if (PER && L.per) min_image(g, rx, ry, rz);
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double r = sqrt(r2);
+        const double qq = r * L.inv_h;
+        if (M.add(live ? q.w : 0.0, r, r2, qq, L.two_h) & live) {
+            if (M.nb < kDensBand) band[M.nb * bstride] = k;
+            M.nb++;
+            M.sb += r < L.two_h ? q.w * m4_shape(qq) : 0.0;
+        }
+    };
+    for_pieces<double, PER, false>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h * (1.0 + kDensBandRel),
+                                   [](double) { return 0.0; },
+                                   [&](uint32_t k0, uint32_t k1, uint32_t, double, double, double, double, double,
+                                       const PieceFrame&) {
+        // slots past the run read the records that follow (kRecordPad zeroed
+        // records past the last particle) and add nothing
+        for (uint32_t k = k0; k < k1; k += 4) {
+            const double4* rec = w.dens_rec + k;
+            const double4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
+            add(q0, k, true);
+            add(q1, k + 1, k + 1 < k1);
+            add(q2, k + 2, k + 2 < k1);
+            add(q3, k + 3, k + 3 < k1);
+        }
+    });
+}
+
+// The band candidates' terms at the lane's current h (FP64 path).
+template <bool PER>
+__device__ double dens_band64(const Work<double>& w, const SegGrid& g, const DensLane<double, PER>& L,
+                              const DensMom& M, const uint32_t* band) {
+    double s = 0.0;
+    const uint32_t bstride = blockDim.x;
+    for (int b = 0; b < M.nb; b++) {
+        const double4 q = w.dens_rec[band[b * bstride]];
+        double rx = L.me.x - q.x, ry = L.me.y - q.y, rz = L.me.z - q.z;
+        if (PER && L.per) min_image(g, rx, ry, rz);
+        const double r = sqrt(rx * rx + ry * ry + rz * rz);
+        s += r < L.two_h ? q.w * m4_shape(r * L.inv_h) : 0.0;
+    }
+    return s;
+}
+
 // The new h from an FP64 density (sph.hpp:195-198), in FP64.
 __device__ __forceinline__ double density_h64(const SegPar& p, double m, double rho) {
     const double h0 = p.h_init;
@@ -668,9 +885,11 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
 // Both iterations of density round r in one pass (sph.hpp:189-201): the sum of
 // particle i uses only its own h, and both iterations of a round share the
 // round's reference grid, so the second iteration needs nothing but the h the
-// first one gives i.  Same arithmetic as k_density run twice; the first
-// iteration's rho and h stay in registers.  Used for every round except round
-// 0 with lists, whose iterations build and read the lists.  The FP32 variant is
+// first one gives i.  The first iteration's rho and h stay in registers.  In
+// FP64 the first iteration is a moment walk and the second walks again only
+// where h left the moments' band (dynamic LDS: dens_band_lds(blockDim.x)
+// bytes); in FP32 both iterations walk.  Used for every round except round 0
+// with lists, whose iterations build and read the lists.  The FP32 variant is
 // held at five waves per SIMD (96 VGPRs, a few epilogue values spilled): at its
 // natural four waves the walk is slower.
 template <class R, bool PER>
@@ -678,6 +897,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
 void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    uint32_t* band = dens_band_slots();
+    (void)band;
+    bool rewalk = false;
     int seg = 0;
     bool act = false;
     if (i < w.n) {
@@ -698,18 +920,42 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
         fin = finite;
         L.h = c.h[i];
         R rho = rho_floor<R>();
+        // FP64: the first iteration is a moment walk (not for lanes that
+        // emulate the reference's cells), and the second walks again only
+        // where h left the moments' band
+        DensMom M;
+        bool mom = false;
         if (finite) {
             L.setup(c, p, g, i, cc);
-            rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
+            if constexpr (sizeof(R) == 8) {
+                if (!L.emu) {
+                    density_walk_mom<PER>(w, g, L, M, band);
+                    rho = smax((M.at(L.h, L.h) + M.sb) * L.sig, rho_floor<R>());
+                    mom = true;
+                }
+            }
+            if (!mom) rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
         }
+        const R h0 = L.h;
         L.h = density_h<R>(c, p, i, rho);
         rho = rho_floor<R>();
         if (finite) {
             L.setup(c, p, g, i, cc);
-            rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
+            bool done = false;
+            if constexpr (sizeof(R) == 8) {
+                if (mom && !L.emu && M.covers(h0, L.h)) {
+                    rho = smax((M.at(h0, L.h) + dens_band64<PER>(w, g, L, M, band)) * L.sig, rho_floor<R>());
+                    done = true;
+                }
+            }
+            if (!done) {
+                rewalk = sizeof(R) == 8;
+                rho = smax(density_walk<R, PER>(c, w, p, g, L), rho_floor<R>());
+            }
         }
         hnew_d = (double)density_store<R, true>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
     }
+    if constexpr (sizeof(R) == 8) count_rewalks(w, seg, act, rewalk);
     raise_h_bound<R>(w, w.grid[seg], cc, hnew_d, act && fin);
     const double v[1] = {hnew_d};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
@@ -721,12 +967,15 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
 // the pressure of the EOS epilogue is evaluated in FP64 from that density.
 // Both iterations need the FP64 terms: an FP32 first iteration moves h by
 // ~3e-7, which the second iteration's density passes on to P at up to ~1e-4
-// of |P| near 10 MPa.  Neighbour lists are not used.
+// of |P| near 10 MPa.  The first iteration is a moment walk, so the second
+// walks again only where h moved out of the moments' band.  Neighbour lists
+// are not used.  Dynamic LDS: dens_band_lds(blockDim.x) bytes.
 template <bool PER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_density_round_hp(Carried<float> c, Work<float> w, int r, int remap,
                                                          SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
+    uint32_t* band = dens_band_slots();
     int seg = 0;
     bool act = false;
     if (i < w.n) {
@@ -735,7 +984,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
     double hnew = 0.0;
     uint32_t cc = 0;
-    bool fin = false;
+    bool fin = false, rewalk = false;
     if (act) {
         const SegPar& p = w.par[seg];
         const SegGrid& g = w.grid[seg];
@@ -750,22 +999,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         H.y = c.y[i];
         H.z = c.z[i];
         const double m = c.mass[i];
-        double h = (double)c.h[i], rho = 1e-30;
-        for (int it = 0; it < 2; it++) {
-            rho = 1e-30;
-            if (finite) {
-                L.h = (float)h;
-                L.setup(c, p, g, i, cc);
-                H.setup(h);
-                rho = smax(density_walk_hp<PER>(c, w, p, g, L, H) * (H.inv_h * H.inv_h * H.inv_h / kPi), 1e-30);
+        const double h0 = (double)c.h[i];
+        double rho = 1e-30;
+        // first iteration: a moment walk, unless the lane emulates the
+        // reference's cells
+        DensMom M;
+        bool mom = false;
+        if (finite) {
+            L.h = (float)h0;
+            L.setup(c, p, g, i, cc);
+            H.setup(h0);
+            double s;
+            if (!L.emu) {
+                density_walk_hp_mom<PER>(w, g, L, H, M, band);
+                s = M.at(h0, h0) + M.sb;
+                mom = true;
+            } else {
+                s = density_walk_hp<PER>(c, w, p, g, L, H);
             }
-            h = density_h64(p, m, rho);
+            rho = smax(s * (H.inv_h * H.inv_h * H.inv_h / kPi), 1e-30);
         }
+        const double h1 = density_h64(p, m, rho);
+        // second iteration: from the moments while h1 stays in their band
+        rho = 1e-30;
+        if (finite) {
+            L.h = (float)h1;
+            L.setup(c, p, g, i, cc);
+            H.setup(h1);
+            double s;
+            if (mom && !L.emu && M.covers(h0, h1)) {
+                s = M.at(h0, h1) + dens_band_hp<PER>(w, g, L, H, M, band);
+            } else {
+                rewalk = true;
+                s = density_walk_hp<PER>(c, w, p, g, L, H);
+            }
+            rho = smax(s * (H.inv_h * H.inv_h * H.inv_h / kPi), 1e-30);
+        }
+        const double h = density_h64(p, m, rho);
         hnew = h;
         w.rho[i] = (float)rho;
         c.h[i] = (float)h;
         eos_epilogue<float, true>(c, w, i, p, info_mat(c.info[i]), (float)rho, (float)h, L.ox, L.oy, L.oz, rho);
     }
+    count_rewalks(w, seg, act, rewalk);
     raise_h_bound<float>(w, w.grid[seg], cc, hnew, act && fin);
     const double v[1] = {hnew};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);

@@ -68,7 +68,9 @@ class Diag(C.Structure):
                 ("dmax", C.c_double), ("t", C.c_double), ("dt_next", C.c_double),
                 ("steps", C.c_int64), ("density_rounds", C.c_int32), ("nonfinite", C.c_int32),
                 ("list_walk_steps", C.c_int64), ("list_overflow", C.c_int64),
-                ("status", C.c_int32), ("pad_", C.c_int32)]
+                ("status", C.c_int32), ("pad_", C.c_int32),
+                ("density_lanes", C.c_int64), ("density_rewalks", C.c_int64),
+                ("density_waves", C.c_int64), ("density_rewalk_waves", C.c_int64)]
 
 
 class Profile(C.Structure):

@@ -110,6 +110,11 @@ typedef struct {
     int64_t list_overflow;      /* particle neighbour lists that did not fit (cumulative)    */
     int32_t status;             /* DSPH_OK, or the dsph_status of the segment's device flags  */
     int32_t pad_;
+    /* Density moment walks (cumulative; counted only when the environment sets
+     * DSPH_COUNT_REWALKS=1 before the context uploads its particles): lanes and
+     * waves of FP64-sum density rounds, and those whose second iteration walked
+     * the cells again because h left the moments' band.  0 otherwise. */
+    int64_t density_lanes, density_rewalks, density_waves, density_rewalk_waves;
 } dsph_diag;
 
 /* Per-phase device time from HIP events (profiling mode). */
