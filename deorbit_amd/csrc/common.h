@@ -55,6 +55,10 @@ struct SegGrid {
     double cw[3], inv_cw[3];     // cell width per axis (periodic axes: L / dim)
     int32_t dim[3];
     uint32_t cell_base, ncells;
+    // bounds of h, one per slice block: the segment's blocks start at
+    // blk_base, column by column, nblk blocks per column
+    uint32_t blk_base;
+    int32_t nblk;
     // periodic axes: the grid spans the domain [dlo, dhi) of length L with
     // dim whole cells, so the image of a cell is exactly L away
     int32_t per[3];
@@ -73,6 +77,7 @@ struct SegState {
     double dt_cur, t;
     long long steps;
     int32_t active[4];
+    int32_t any_active[4];                 // segment 0 only: does any segment run round r
     int32_t rounds;
     int32_t bad;                           // non-zero: capacity, numeric or reach trouble
     double hmax_final;
@@ -122,7 +127,7 @@ struct Work {
     uint32_t* perm;                  // new slot -> slot in the order before this step's sort
     uint32_t* cstart;                // per sort cell c: first slot of c in the current order, with one
                                      // entry past the last cell, so cell c is [cstart[c], cstart[c+1])
-    float* cell_hmax;                // at the key of a block's first slice: upper bound of h in the block
+    float* cell_hmax;                // per slice block (block_key): upper bound of h in the block
     // Neighbour lists, built by the first density iteration of a step and read
     // by the second iteration and the force pass.  Entries come in quads of
     // four indices; quad q of particle i sits at
@@ -372,9 +377,11 @@ __device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
 __device__ __host__ __forceinline__ uint32_t cell_key(const SegGrid& g, int cx, int cy, int cz) {
     return g.cell_base + ((uint32_t)cx * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[2] + (uint32_t)cz;
 }
-// Key where the bound of h of the block holding slice cz lives.
+// Index of the bound of h of the block holding slice cz of column (cx, cy)
+// (Work::cell_hmax).
 __device__ __host__ __forceinline__ uint32_t block_key(const SegGrid& g, int cx, int cy, int cz) {
-    return cell_key(g, cx, cy, block_start(cz));
+    return g.blk_base + ((uint32_t)cx * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.nblk +
+           (uint32_t)(cz / kSliceBlock);
 }
 __device__ __host__ __forceinline__ uint32_t cellc_pack(int cx, int cy, int cz) {
     return (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);

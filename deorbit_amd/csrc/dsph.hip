@@ -176,8 +176,8 @@ class Engine : public EngineBase {
         for (int k = 0; k < 2; k++) { keys_[k] = dalloc<uint32_t>(n_); vals_[k] = dalloc<uint32_t>(n_); }
         hist_ = dalloc<uint32_t>((size_t)256 * nb_sort_);
         digit_totals_ = dalloc<uint32_t>(256);
-        d_total_cells_ = dalloc<uint32_t>(2);
-        zero(d_total_cells_, 2);
+        d_total_cells_ = dalloc<uint32_t>(3);   // cells laid out, cells wanted, slice blocks laid out
+        zero(d_total_cells_, 3);
         // per-segment tables
         std::vector<MatT<R>> hm(mats_.size());
         for (size_t k = 0; k < mats_.size(); k++) hm[k] = to_mat(mats_[k]);

@@ -106,17 +106,17 @@ __global__ __launch_bounds__(256) void k_kick_drift(Carried<R> c, Work<R> w, int
 constexpr int kGridThreads = 1024;
 __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* st, const SegPar* par, int nseg,
                                                        double cell_factor, uint32_t capacity, uint32_t* total_out) {
-    __shared__ unsigned long long s_scan[kGridThreads];
-    __shared__ unsigned long long s_carry;
+    __shared__ unsigned long long s_scan[kGridThreads], s_scanb[kGridThreads];
+    __shared__ unsigned long long s_carry, s_carryb;
     const int t = threadIdx.x;
     double scale = 1.0;
-    unsigned long long total = 0, wanted = 0;
+    unsigned long long total = 0, total_blk = 0, wanted = 0;
     for (int attempt = 0; attempt < 200; attempt++) {
-        if (t == 0) s_carry = 0;
+        if (t == 0) { s_carry = 0; s_carryb = 0; }
         __syncthreads();
         for (int s0 = 0; s0 < nseg; s0 += kGridThreads) {
             const int s = s0 + t;
-            unsigned long long nc = 0;
+            unsigned long long nc = 0, nbk = 0;
             if (s < nseg) {
                 SegGrid& g = grid[s];
                 double lo[3], ext[3];
@@ -163,29 +163,43 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
                 }
                 nc += (unsigned long long)g.dim[0] * (unsigned long long)g.dim[1] * (unsigned long long)g.dim[2];
                 g.ncells = (uint32_t)nc;
+                g.nblk = (g.dim[2] + kSliceBlock - 1) / kSliceBlock;
+                nbk = (unsigned long long)g.dim[0] * (unsigned long long)g.dim[1] * (unsigned long long)g.nblk;
             }
-            // inclusive scan of the chunk's cell counts
+            // inclusive scans of the chunk's cell and block counts
             s_scan[t] = nc;
+            s_scanb[t] = nbk;
             __syncthreads();
             for (int o = 1; o < kGridThreads; o <<= 1) {
                 const unsigned long long add = t >= o ? s_scan[t - o] : 0ull;
+                const unsigned long long addb = t >= o ? s_scanb[t - o] : 0ull;
                 __syncthreads();
                 s_scan[t] += add;
+                s_scanb[t] += addb;
                 __syncthreads();
             }
-            const unsigned long long carry = s_carry;
-            if (s < nseg) grid[s].cell_base = (uint32_t)(carry + s_scan[t] - nc);
+            const unsigned long long carry = s_carry, carryb = s_carryb;
+            if (s < nseg) {
+                grid[s].cell_base = (uint32_t)(carry + s_scan[t] - nc);
+                grid[s].blk_base = (uint32_t)(carryb + s_scanb[t] - nbk);
+            }
             __syncthreads();
-            if (t == kGridThreads - 1) s_carry = carry + s_scan[t];
+            if (t == kGridThreads - 1) {
+                s_carry = carry + s_scan[t];
+                s_carryb = carryb + s_scanb[t];
+            }
             __syncthreads();
         }
         total = s_carry;
+        total_blk = s_carryb;
         __syncthreads();
         if (attempt == 0) wanted = total;
         if (total <= capacity) break;
         scale *= 1.25;
     }
     if (t == 0) {
+        // slice blocks of the grid laid out (never more than its cells)
+        total_out[2] = (uint32_t)(total_blk < capacity ? total_blk : capacity);
         if (total > capacity) { st[0].bad |= 1; total = capacity; }
         total_out[0] = (uint32_t)total;
         total_out[1] = wanted > 0xffffffffull ? 0xffffffffu : (uint32_t)wanted;
@@ -217,7 +231,8 @@ __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t*
         vals[o] = o;
         w.dev_of[o] = (uint32_t)i;
     }
-    const uint32_t total = *total_cells;
+    // clear the bounds of h of every slice block (total_cells[2] of them)
+    const uint32_t total = total_cells[2];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t k = (uint32_t)(blockIdx.x * blockDim.x + threadIdx.x); k < total; k += stride)
         w.cell_hmax[k] = 0.0f;
@@ -497,6 +512,9 @@ __global__ __launch_bounds__(256) void k_permute(Carried<R> src, Carried<R> dst,
 // reference would build for it; r == 4: record the rounds run and the final
 // h_max for the force stencils.
 __global__ void k_round(SegState* st, SegGrid* grid, const SegPar* par, int nseg, int r) {
+    __shared__ int s_any;
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
         SegState& S = st[s];
         if (r < 4) {
@@ -506,6 +524,7 @@ __global__ void k_round(SegState* st, SegGrid* grid, const SegPar* par, int nseg
                 act = S.active[r - 1] && !(hm <= h0 * 1.02);
             }
             S.active[r] = act;
+            if (act) atomicOr(&s_any, 1);
             if (act) {
                 SegGrid& g = grid[s];
                 const double hmax = fmax(bits_to_double(S.hmax[r]), 1e-30);
@@ -536,6 +555,8 @@ __global__ void k_round(SegState* st, SegGrid* grid, const SegPar* par, int nseg
             }
         }
     }
+    __syncthreads();
+    if (r < 4 && threadIdx.x == 0) st[0].any_active[r] = s_any;
 }
 
 }  // namespace dsph

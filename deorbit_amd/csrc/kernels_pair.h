@@ -390,8 +390,9 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                 R rb = reach_max;
                 if constexpr (HB) {
                     // the larger bound of h of the (at most two) blocks met
-                    const R hb0 = (R)w.cell_hmax[col + f0];
-                    const R hb1 = (f1 != f0) ? (R)w.cell_hmax[col + f1] : hb0;
+                    const uint32_t cb = block_key(g, nx, ny, 0);
+                    const R hb0 = (R)w.cell_hmax[cb + f0 / kSliceBlock];
+                    const R hb1 = (f1 != f0) ? (R)w.cell_hmax[cb + f1 / kSliceBlock] : hb0;
                     rb = reach_of(hb0 > hb1 ? hb0 : hb1);
                     const R rb2 = rb * rb;
                     if (gxy >= rb2) continue;
@@ -596,8 +597,8 @@ __device__ double density_walk_hp(const Carried<float>& c, const Work<float>& w,
 // the shape itself does at q = 1.  A particle whose h1 leaves the band, whose
 // bands held more than kDensBand candidates, or whose 2 h1 may exceed the
 // reference's cell width (emulation) walks again.
-constexpr int kDensBand = 8;
-constexpr double kDensBandRel = 5e-3;
+constexpr int kDensBand = 16;
+constexpr double kDensBandRel = 1e-2;
 
 struct DensMom {
     double a0, a2, a3, b0, b1, b2, b3;
@@ -882,6 +883,16 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
     }
 }
 
+// A density round that no segment runs: the whole workgroup leaves at once,
+// after marking its partial-reduction record empty.  Every thread of the
+// workgroup reads the same flag.
+template <class R>
+__device__ __forceinline__ bool round_idle(const Work<R>& w, int r) {
+    if (w.st[0].any_active[r] != 0) return false;
+    if (threadIdx.x == 0) w.parts[blockIdx.x].seg = -1;
+    return true;
+}
+
 // Both iterations of density round r in one pass (sph.hpp:189-201): the sum of
 // particle i uses only its own h, and both iterations of a round share the
 // round's reference grid, so the second iteration needs nothing but the h the
@@ -895,6 +906,7 @@ __global__ __launch_bounds__(256) void k_density(Carried<R> c, Work<R> w, int r,
 template <class R, bool PER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
 void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTargets<1> tg) {
+    if (round_idle(w, r)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
     uint32_t* band = dens_band_slots();
@@ -973,6 +985,7 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
 template <bool PER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_density_round_hp(Carried<float> c, Work<float> w, int r, int remap,
                                                          SegTargets<1> tg) {
+    if (round_idle(w, r)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
     uint32_t* band = dens_band_slots();
