@@ -59,6 +59,8 @@ struct SegGrid {
     // blk_base, column by column, nblk blocks per column
     uint32_t blk_base;
     int32_t nblk;
+    int32_t tiles_y;     // column tiles along y (col_index)
+    int32_t tile_sh;     // column tiles are 2^tile_sh columns wide
     // periodic axes: the grid spans the domain [dlo, dhi) of length L with
     // dim whole cells, so the image of a cell is exactly L away
     int32_t per[3];
@@ -373,15 +375,26 @@ __device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
     return x * q + (x < r ? x : r) + k;
 }
 
-// Column-major key: columns x-major then y, slices z fastest within a column.
+// Columns are numbered in square tiles of 2^tile_sh x 2^tile_sh columns
+// (tiles x-major over the grid, columns x-major inside a tile; the grid is
+// padded to whole tiles with empty columns), so the columns around a particle,
+// which a stencil walk visits, lie close together in memory and the neighbour
+// records of the waves in flight on one XCD share its L2.  k_grid picks the
+// largest tile up to kColTileMax columns wide whose padding stays within a
+// tenth of the segment's smaller column count.
+constexpr int kColTileMax = 16;
+__device__ __host__ __forceinline__ uint32_t col_index(const SegGrid& g, int cx, int cy) {
+    const uint32_t ux = (uint32_t)cx, uy = (uint32_t)cy, sh = (uint32_t)g.tile_sh, m = (1u << sh) - 1u;
+    return (((((ux >> sh) * (uint32_t)g.tiles_y + (uy >> sh)) << sh) + (ux & m)) << sh) + (uy & m);
+}
+// Column-major key: columns in tile order, slices z fastest within a column.
 __device__ __host__ __forceinline__ uint32_t cell_key(const SegGrid& g, int cx, int cy, int cz) {
-    return g.cell_base + ((uint32_t)cx * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[2] + (uint32_t)cz;
+    return g.cell_base + col_index(g, cx, cy) * (uint32_t)g.dim[2] + (uint32_t)cz;
 }
 // Index of the bound of h of the block holding slice cz of column (cx, cy)
 // (Work::cell_hmax).
 __device__ __host__ __forceinline__ uint32_t block_key(const SegGrid& g, int cx, int cy, int cz) {
-    return g.blk_base + ((uint32_t)cx * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.nblk +
-           (uint32_t)(cz / kSliceBlock);
+    return g.blk_base + col_index(g, cx, cy) * (uint32_t)g.nblk + (uint32_t)(cz / kSliceBlock);
 }
 __device__ __host__ __forceinline__ uint32_t cellc_pack(int cx, int cy, int cz) {
     return (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);

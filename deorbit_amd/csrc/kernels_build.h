@@ -161,10 +161,19 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(SegGrid* grid, SegState* 
                     }
                     g.inv_cw[a] = 1.0 / g.cw[a];
                 }
-                nc += (unsigned long long)g.dim[0] * (unsigned long long)g.dim[1] * (unsigned long long)g.dim[2];
+                // columns padded to whole tiles (col_index)
+                const int dmin = g.dim[0] < g.dim[1] ? g.dim[0] : g.dim[1];
+                int sh = 0;
+                while ((2 << sh) <= kColTileMax && 10 * (2 << sh) <= dmin) sh++;
+                const int T = 1 << sh;
+                g.tile_sh = sh;
+                const int tx = (g.dim[0] + T - 1) >> sh;
+                g.tiles_y = (g.dim[1] + T - 1) >> sh;
+                const unsigned long long ncol = (unsigned long long)tx * g.tiles_y * T * T;
+                nc += ncol * (unsigned long long)g.dim[2];
                 g.ncells = (uint32_t)nc;
                 g.nblk = (g.dim[2] + kSliceBlock - 1) / kSliceBlock;
-                nbk = (unsigned long long)g.dim[0] * (unsigned long long)g.dim[1] * (unsigned long long)g.nblk;
+                nbk = ncol * (unsigned long long)g.nblk;
             }
             // inclusive scans of the chunk's cell and block counts
             s_scan[t] = nc;
