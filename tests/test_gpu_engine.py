@@ -387,3 +387,40 @@ def test_single_particle_density_has_the_closed_form(precision, tol, gpu):
     s.synchronize()
     assert np.all(s.get("acc") == 0.0) and s.diagnostics()["nonfinite"] == 0
     s.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(dsph.FP64, 1e-13), (dsph.MIXED, 1e-6)])
+def test_density_moment_walks_and_rewalks_match_the_oracle(precision, tol, gpu, monkeypatch):
+    """The second density iteration of a round comes from the first walk's moment
+    sums while h stays in their band and from a second walk elsewhere; both
+    branches run here (init moves h far from h_init, later steps little) and the
+    densities and smoothing lengths follow the oracle's direct sums (sph.hpp:189-198)
+    per particle.  The counters (DSPH_COUNT_REWALKS=1) show which branch ran."""
+    monkeypatch.setenv("DSPH_COUNT_REWALKS", "1")
+    ic = ics.impact_37k()
+    o = po.make_system(ic, ic["params"], ic["materials"], dx=ic["dx"])
+    o.init()
+    s = dsph.make_system(ic, precision)
+    s.init()
+    d0 = s.diagnostics()
+    n = len(ic["mass"])
+    assert d0["density_lanes"] >= n and d0["density_rewalks"] > 0, d0
+    for k in range(8):
+        dt = o.timestep()
+        o.step(dt)
+        s.step(dt)
+        e_rho = float(np.max(np.abs(s.get("rho") - o.get(po.RHO)) / o.get(po.RHO)))
+        e_h = float(np.max(np.abs(s.get("hh") - o.get(po.HH)) / o.get(po.HH)))
+        assert e_rho < tol and e_h < tol, (k, e_rho, e_h)
+    d1 = s.diagnostics()
+    lanes = d1["density_lanes"] - d0["density_lanes"]
+    rewalks = d1["density_rewalks"] - d0["density_rewalks"]
+    print(f"moment walks: init {d0['density_rewalks']}/{d0['density_lanes']} lanes walked again, "
+          f"8 steps {rewalks}/{lanes}")
+    assert lanes >= 8 * n and rewalks < 0.5 * lanes, (lanes, rewalks)
+    s.close()
+    monkeypatch.delenv("DSPH_COUNT_REWALKS")
+    s2 = dsph.make_system(ic, precision)           # counters stay off without the variable
+    s2.init()
+    assert s2.diagnostics()["density_lanes"] == 0
+    s2.close()
