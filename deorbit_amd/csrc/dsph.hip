@@ -281,6 +281,7 @@ class Engine : public EngineBase {
         capacity_ = (uint32_t)cap;
         w_.cstart = dalloc<uint32_t>((size_t)capacity_ + 1);
         w_.cell_hmax = dalloc<float>(capacity_);
+        chunk_lb_ = dalloc<uint32_t>((size_t)capacity_ / kStartChunk + 3);
         int bits = 1;
         while (bits < 32 && ((1ull << bits) <= (unsigned long long)capacity_)) bits++;
         passes_ = (bits + 7) / 8;
@@ -444,9 +445,11 @@ class Engine : public EngineBase {
             ka = 1 - ka;
         }
         // first slot of every cell (empty cells included), from the sorted keys
-        const unsigned ncs = std::min<unsigned>(blocks_for((int64_t)capacity_ + 1, 256 * kCellChunk), 2048u);
+        launch(PH_SORT, "chunk_start", k_chunk_start, dim3(std::min<unsigned>(blocks_for(n_ + 1, 256), 4096u)),
+               dim3(256), (const uint32_t*)keys_[ka], n_, chunk_lb_, (const uint32_t*)d_total_cells_);
+        const unsigned ncs = std::min<unsigned>(blocks_for((int64_t)capacity_ + 1, 256 * kCellChunk), 1u << 20);
         launch(PH_SORT, "cell_start", k_cell_start, dim3(ncs), dim3(256), (const uint32_t*)keys_[ka], n_, w_.cstart,
-               (const uint32_t*)d_total_cells_);
+               (const uint32_t*)chunk_lb_, (const uint32_t*)d_total_cells_);
         const auto th = hmax_target(0);
         launch(PH_PERMUTE, "permute", k_permute<R>, dim3(blocks_for(n_, bp)), dim3(bp), bufs_[cur_],
                bufs_[1 - cur_], w_, (const uint32_t*)keys_[ka], (const uint32_t*)vals_[ka], th, w_.nbr_cap > 0 ? 1 : 0);
@@ -725,14 +728,14 @@ class Engine : public EngineBase {
                                               std::min<double>(kMaxCellsPerParticle, std::ceil(want)));
         if (per == cur || (per < cur && !shrink)) return false;
         release_graphs();
-        for (void* old : {(void*)w_.cstart, (void*)w_.cell_hmax}) free_ptr(old);
+        for (void* old : {(void*)w_.cstart, (void*)w_.cell_hmax, (void*)chunk_lb_}) free_ptr(old);
         tune_.cell_capacity_per_particle = per;
         try {
             alloc_table();
         } catch (const Err&) {
             // not enough memory: back to a table of the old size (coarser cells)
             (void)hipGetLastError();
-            free_ptr(w_.cstart);
+            for (void* part : {(void*)w_.cstart, (void*)w_.cell_hmax, (void*)chunk_lb_}) free_ptr(part);
             tune_.cell_capacity_per_particle = cur;
             alloc_table();
             return false;
@@ -972,7 +975,7 @@ class Engine : public EngineBase {
                 tune_ = nt;
                 // the table is reallocated; everything else is kept
                 release_graphs();
-                for (void* old : {(void*)w_.cstart, (void*)w_.cell_hmax}) free_ptr(old);
+                for (void* old : {(void*)w_.cstart, (void*)w_.cell_hmax, (void*)chunk_lb_}) free_ptr(old);
                 alloc_table();
                 built_ = false;
                 records_ = false;
@@ -1151,6 +1154,7 @@ class Engine : public EngineBase {
     uint32_t* d_total_cells_ = nullptr;
     int nb_sort_ = 0, passes_ = 0, nparts_max_ = 0;
     uint32_t capacity_ = 0;
+    uint32_t* chunk_lb_ = nullptr;   // first slot of every chunk of kStartChunk cells (k_chunk_start)
     MatT<R>* d_mats_ = nullptr;
     MatT<double>* d_mats64_ = nullptr;
     bool density64_ = false;

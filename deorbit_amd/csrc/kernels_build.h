@@ -248,25 +248,61 @@ __global__ __launch_bounds__(256) void k_keys(Carried<R> c, Work<R> w, uint32_t*
 }
 
 // First slot of every cell in the sorted keys (a lower bound per cell, so empty
-// cells get the slot where they would start), for cells 0 .. total inclusive.
-// A thread takes kCellChunk consecutive cells: one binary search for the first,
-// then a merge walk along the keys for the rest.
+// cells get the slot where they would start), for cells 0 .. total inclusive,
+// in two passes.  k_chunk_start records the first slot of every chunk of
+// kStartChunk cells: slot i, between keys k[i-1] and k[i] (k[-1] = -inf,
+// k[n] = +inf), writes the entries of the chunks that start in (k[i-1], k[i]],
+// so every entry is written once.  k_cell_start then gives each wave one chunk
+// and each lane kCellChunk consecutive cells of it: a binary search between the
+// chunk's first slot and the next chunk's for the lane's first cell, then a
+// merge walk along the keys for the rest.
 constexpr int kCellChunk = 8;
+static_assert(kCellChunk == 8, "k_cell_start stores a lane's cells as two 16-byte vectors");
+constexpr int kStartChunk = 64 * kCellChunk;
+__device__ __forceinline__ int64_t lower_bound_key(const uint32_t* skeys, int64_t lo, int64_t hi, uint64_t c) {
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (skeys[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+__global__ __launch_bounds__(256) void k_chunk_start(const uint32_t* skeys, int64_t n, uint32_t* lb,
+                                                    const uint32_t* total_cells) {
+    const int64_t last_chunk = (int64_t)(*total_cells / kStartChunk) + 1;    // entries 0 .. last_chunk
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        const int64_t j0 = i == 0 ? 0 : (int64_t)(skeys[i - 1] / kStartChunk) + 1;
+        const int64_t j1 = i == n ? last_chunk : (int64_t)(skeys[i] / kStartChunk);
+        for (int64_t j = j0; j <= j1; j++) lb[j] = (uint32_t)i;
+    }
+}
 __global__ __launch_bounds__(256) void k_cell_start(const uint32_t* skeys, int64_t n, uint32_t* cstart,
-                                                   const uint32_t* total_cells) {
+                                                   const uint32_t* lb, const uint32_t* total_cells) {
     const uint64_t total = *total_cells;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kCellChunk;
-    for (uint64_t c0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kCellChunk; c0 <= total; c0 += stride) {
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (skeys[mid] < c0) lo = mid + 1;
-            else hi = mid;
-        }
-        const uint64_t c1 = c0 + kCellChunk < total + 1 ? c0 + kCellChunk : total + 1;
-        for (uint64_t c = c0; c < c1; c++) {
-            while (lo < n && skeys[lo] < c) lo++;
-            cstart[c] = (uint32_t)lo;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwave = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t j = wave; j * kStartChunk <= total; j += nwave) {
+        const uint64_t c0 = j * kStartChunk + (uint64_t)lane * kCellChunk;
+        if (c0 > total) continue;
+        int64_t a = lower_bound_key(skeys, lb[j], lb[j + 1], c0);
+        if (c0 + kCellChunk <= total + 1) {
+            // a lane's cells are 32-byte aligned: two 16-byte stores
+            uint32_t v[kCellChunk];
+#pragma unroll
+            for (int k = 0; k < kCellChunk; k++) {
+                while (a < n && skeys[a] < c0 + k) a++;
+                v[k] = (uint32_t)a;
+            }
+            uint4* dst = (uint4*)(cstart + c0);
+            dst[0] = make_uint4(v[0], v[1], v[2], v[3]);
+            dst[1] = make_uint4(v[4], v[5], v[6], v[7]);
+        } else {
+            for (uint64_t c = c0; c < total + 1; c++) {
+                while (a < n && skeys[a] < c) a++;
+                cstart[c] = (uint32_t)a;
+            }
         }
     }
 }
