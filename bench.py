@@ -281,12 +281,14 @@ def main():
     pdir = os.path.join(ROOT, "profiles")
     ps = args.profile_steps
     # the force pass over neighbour lists or by cell walks (whichever the tuning
-    # selected); the benchmark workloads have no periodic axis: non-periodic variants
+    # selected); the benchmark workloads have no periodic axis: non-periodic variants,
+    # instantiated for one segment except on the batched ensemble
     fphase = "force_list" if prof.get("force_list", (0.0, 0))[0] > prof["force"][0] else "force"
+    one = "false" if args.workload == "ensemble_256" else "true"
     kernels = {
-        "force": dict(kernel="k_force_list<float, false>" if fphase == "force_list" else "k_force<float, false>",
+        "force": dict(kernel="k_force_list<float, false>" if fphase == "force_list" else f"k_force<float, false, {one}>",
                       phase=fphase, bytes_pp=roofline.kernel_bytes("mixed")["force"], flops_pp=roofline.force_flops()),
-        "density": dict(kernel="k_density_round_hp<false>", phase="density",
+        "density": dict(kernel=f"k_density_round_hp<false, {one}>", phase="density",
                         bytes_pp=roofline.density_round_bytes("mixed", True), flops_pp=roofline.density_flops()),
     }
     for key, k in kernels.items():

@@ -496,8 +496,12 @@ class Engine : public EngineBase {
             const auto th = hmax_target(r + 1);
             if constexpr (sizeof(R) == 4) {
                 if (density64_) {
-                    launch_lds(PH_DENSITY, "density", k_density_round_hp<PER>, dim3(nb), dim3(bq),
-                               dens_band_lds(bq), cur(), w_, r, tune_.xcd_remap, th);
+                    if (nseg_ == 1)
+                        launch_lds(PH_DENSITY, "density", k_density_round_hp<PER, true>, dim3(nb), dim3(bq),
+                                   dens_band_lds(bq), cur(), w_, r, tune_.xcd_remap, th);
+                    else
+                        launch_lds(PH_DENSITY, "density", k_density_round_hp<PER, false>, dim3(nb), dim3(bq),
+                                   dens_band_lds(bq), cur(), w_, r, tune_.xcd_remap, th);
                     fold_parts(PH_DENSITY, th, nb);
                     continue;
                 }
@@ -508,8 +512,13 @@ class Engine : public EngineBase {
                 launch(PH_DENSITY, "density", k_density<R, true, PER>, dim3(nb), dim3(bq), cur(), w_, r,
                        tune_.xcd_remap, lists, th);
             } else {
-                launch_lds(PH_DENSITY, "density", k_density_round<R, PER>, dim3(nb), dim3(bq),
-                           sizeof(R) == 8 ? dens_band_lds(bq) : 0, cur(), w_, r, tune_.xcd_remap, lists, th);
+                const size_t lds = sizeof(R) == 8 ? dens_band_lds(bq) : 0;
+                if (nseg_ == 1)
+                    launch_lds(PH_DENSITY, "density", k_density_round<R, PER, true>, dim3(nb), dim3(bq), lds, cur(),
+                               w_, r, tune_.xcd_remap, lists, th);
+                else
+                    launch_lds(PH_DENSITY, "density", k_density_round<R, PER, false>, dim3(nb), dim3(bq), lds,
+                               cur(), w_, r, tune_.xcd_remap, lists, th);
             }
             fold_parts(PH_DENSITY, th, nb);
             if (r == 0 && lists) {
@@ -537,7 +546,11 @@ class Engine : public EngineBase {
         launch(PH_ROUND, "reset_dt", k_reset_dt, dim3(1), dim3(256), d_st_, nseg_);
         const unsigned nb = blocks_for(n_, bq);
         const int lists = lists_built_ ? 1 : 0;
-        launch(PH_FORCE, "force", k_force<R, PER>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, lists, td);
+        if (nseg_ == 1)
+            launch(PH_FORCE, "force", k_force<R, PER, true>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, lists, td);
+        else
+            launch(PH_FORCE, "force", k_force<R, PER, false>, dim3(nb), dim3(bq), cur(), w_, tune_.xcd_remap, lists,
+                   td);
         if (lists) {
             fold_parts(PH_FORCE, td, nb);
             launch(PH_FORCE_LIST, "force_list", k_force_list<R, PER>, dim3(nb), dim3(bq), cur(), w_,

@@ -848,8 +848,11 @@ __device__ __forceinline__ bool round_idle(const Work<R>& w, int r) {
 // bytes); in FP32 both iterations walk.  Used for every round except round 0
 // with lists, whose iterations build and read the lists.  The FP32 variant is
 // held at five waves per SIMD (96 VGPRs, a few epilogue values spilled): at its
-// natural four waves the walk is slower.
-template <class R, bool PER>
+// natural four waves the walk is slower.  SEG1 (here, in k_density_round_hp and
+// in k_force): the context holds one segment, so its parameters and grid are
+// read at index 0, which the compiler keeps in scalar registers instead of one
+// copy per lane (4 % faster density, 1 % faster force pass at 4M particles).
+template <class R, bool PER, bool SEG1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
 void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTargets<1> tg) {
     if (round_idle(w, r)) return;
@@ -868,8 +871,8 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
     uint32_t cc = 0;
     bool fin = false;
     if (act) {
-        const SegPar& p = w.par[seg];
-        const SegGrid& g = w.grid[seg];
+        const SegPar& p = w.par[SEG1 ? 0 : seg];
+        const SegGrid& g = w.grid[SEG1 ? 0 : seg];
         DensLane<R, PER> L;
         cc = w.cellc[i];
         L.me = w.dens_rec[i];
@@ -914,7 +917,7 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
         hnew_d = (double)density_store<R, true>(c, w, i, seg, p, rho, finite, lists, L.ox, L.oy, L.oz);
     }
     if constexpr (sizeof(R) == 8) count_rewalks(w, seg, act, rewalk);
-    raise_h_bound<R>(w, w.grid[seg], cc, hnew_d, act && fin);
+    raise_h_bound<R>(w, w.grid[SEG1 ? 0 : seg], cc, hnew_d, act && fin);
     const double v[1] = {hnew_d};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
 }
@@ -928,7 +931,7 @@ void k_density_round(Carried<R> c, Work<R> w, int r, int remap, int lists, SegTa
 // of |P| near 10 MPa.  The first iteration is a moment walk, so the second
 // walks again only where h moved out of the moments' band.  Neighbour lists
 // are not used.  Dynamic LDS: dens_band_lds(blockDim.x) bytes.
-template <bool PER>
+template <bool PER, bool SEG1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_density_round_hp(Carried<float> c, Work<float> w, int r, int remap,
                                                          SegTargets<1> tg) {
     if (round_idle(w, r)) return;
@@ -945,8 +948,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     uint32_t cc = 0;
     bool fin = false, rewalk = false;
     if (act) {
-        const SegPar& p = w.par[seg];
-        const SegGrid& g = w.grid[seg];
+        const SegPar& p = w.par[SEG1 ? 0 : seg];
+        const SegGrid& g = w.grid[SEG1 ? 0 : seg];
         DensLane<float, PER> L;
         DensHP H;
         cc = w.cellc[i];
@@ -1001,7 +1004,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         eos_epilogue<float, true>(c, w, i, p, info_mat(c.info[i]), (float)rho, (float)h, L.ox, L.oy, L.oz, rho);
     }
     count_rewalks(w, seg, act, rewalk);
-    raise_h_bound<float>(w, w.grid[seg], cc, hnew, act && fin);
+    raise_h_bound<float>(w, w.grid[SEG1 ? 0 : seg], cc, hnew, act && fin);
     const double v[1] = {hnew};
     block_reduce<1>(tg, sizeof(SegState) / 8, w.parts, seg, v, act);
 }
@@ -1341,7 +1344,7 @@ __device__ __forceinline__ bool force_list_usable(const Work<R>& w, int64_t i, i
 // per-block bound of h), in the mirrored column order (for_pieces, MIR).  With
 // lists built, particles whose list is usable are left to k_force_list, which
 // sums in the list's order (the density walk's).
-template <class R, bool PER>
+template <class R, bool PER, bool SEG1>
 __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int remap, int lists, SegTargets<1> tg) {
     const int b = xcd_block(blockIdx.x, gridDim.x, remap);
     const int64_t i = (int64_t)b * blockDim.x + threadIdx.x;
@@ -1353,8 +1356,8 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
         seg = info_seg(inf);
         mine = !force_list_usable<R>(w, i, seg, lists);
         if (mine) {
-            const SegPar& p = w.par[seg];
-            const SegGrid& g = w.grid[seg];
+            const SegPar& p = w.par[SEG1 ? 0 : seg];
+            const SegGrid& g = w.grid[SEG1 ? 0 : seg];
             const uint32_t cc = w.cellc[i];
             ForceLane<R, PER> F;
             F.setup(c, w, p, g, i, cc);
@@ -1362,7 +1365,7 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
             F.ozs = slice_offset<R>(g, F.cz, F.oz);
             if (isfinite((double)F.ox) && isfinite((double)F.oy) && isfinite((double)F.oz)) {
                 for_pieces<R, PER, true, true>(g, w, F.cx, F.cy, F.cz, F.ox, F.oy, F.ozs,
-                                               F.hi + (R)w.st[seg].hmax_final, [&](R hb) { return F.hi + hb; },
+                                               F.hi + (R)w.st[SEG1 ? 0 : seg].hmax_final, [&](R hb) { return F.hi + hb; },
                                          [&](uint32_t k0, uint32_t k1, uint32_t ks, R sx, R sy, R sz0, R sz1,
                                              R, const PieceFrame&) {
                     R bx, by, bz0, bz1;
