@@ -47,13 +47,16 @@ typedef enum {
 
 /* DSPH_MIXED: FP32 force-pass pair sums, stress and sound speed; density pair
  *   terms, density, smoothing length and pressure in FP64 (the per-particle
- *   pressure follows the FP64 reference to ~1e-7 relative above 10 MPa);
+ *   pressure above 10 MPa follows the FP64 reference to ~1e-6 relative after a
+ *   step and within 1e-4 through ten steps);
  *   positions, velocities, internal energy, time step and reductions in FP64.
  * DSPH_FP64: every quantity in FP64 (the comparator path).
  * DSPH_MIXED_FAST: DSPH_MIXED with FP32 density sums and pressure as well
  *   (density within ~1e-6, pressure within ~1e-6 of max|P| but only ~5e-3 per
  *   particle where |rho/rho0 - 1| is small); the only mixed mode that can use
- *   neighbour lists. */
+ *   neighbour lists.  It walks the neighbours twice per density round where
+ *   the FP64 sums of DSPH_MIXED walk once, so it is at most a few percent
+ *   faster. */
 typedef enum { DSPH_MIXED = 0, DSPH_FP64 = 1, DSPH_MIXED_FAST = 2 } dsph_precision;
 typedef enum { DSPH_EOS_IDEAL = 0, DSPH_EOS_TILLOTSON = 1 } dsph_eos_type;
 
