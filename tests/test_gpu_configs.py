@@ -301,21 +301,16 @@ def test_free_running_impact_matches_a_free_running_oracle(precision, gpu):
     s.close()
 
 
-def test_layered_4m_whole_run_mixed_follows_fp64(gpu):
-    """The whole 1000-step layered_4m run (4.0M particles), free-running: the FP64
-    device path, which follows the oracle to ~1e-14 on this workload (above),
-    sets the end time with its 1000 CFL steps; DSPH_MIXED runs to the same time
-    with its own steps.  The impactor aggregates the reference driver prints
-    agree in centroid and mean speed to ~1e-6; the melt fraction (u > 1 MJ/kg)
-    and the largest u of the impactor are extremes of single particles in a
-    violent contact and are bounded loosely (measured 1.2 % and 4 %)."""
-    ic = ics.layered_4m()
+def whole_run_mixed_vs_fp64(ic, steps):
+    """The FP64 device path runs `steps` CFL steps and sets the end time; DSPH_MIXED
+    runs to the same time with its own steps.  Returns both sets of impactor
+    aggregates and the mixed path's step count."""
     s = dsph.make_system(ic, dsph.FP64)
     s.init()
-    for _ in range(20):
+    for _ in range(steps // 50):
         s.run(50, 0.0, True)
     d = s.diagnostics()
-    assert d["status"] == 0 and d["steps"] == 1000
+    assert d["status"] == 0 and d["steps"] == steps
     t_end = d["t"]
     a64 = impactor_aggregates(device_fields(s), ic["mat"])
     s.close()
@@ -324,17 +319,43 @@ def test_layered_4m_whole_run_mixed_follows_fp64(gpu):
     m.add_ic(ic)
     m.seed_damage(ic["dx"], 12345)
     m.init()
-    while m.diagnostics()["steps"] < 950:
+    while m.diagnostics()["steps"] < steps - 50:
         m.run(50, 0.0, True)
-    while m.diagnostics()["t"] < t_end and m.diagnostics()["steps"] < 1100:
+    while m.diagnostics()["t"] < t_end and m.diagnostics()["steps"] < steps + 100:
         m.run(1, 0.0, True)
     d = m.diagnostics()
     assert d["t"] == t_end and d["status"] == 0
     a = impactor_aggregates(device_fields(m), ic["mat"])
     m.close()
-    print(f"layered_4m whole run: mixed {d['steps']} steps, fp64 1000 steps to t={t_end:.6e} s; " +
+    print(f"{ic['meta']['name']} whole run: mixed {d['steps']} steps, fp64 {steps} steps to t={t_end:.6e} s; " +
           " ".join(f"{k} {a[k]:.6g}/{a64[k]:.6g}" for k in a), flush=True)
-    assert abs(d["steps"] - 1000) <= 20
+    return a, a64, d["steps"]
+
+
+def test_graze_1m_whole_run_mixed_follows_fp64(gpu):
+    """The whole 500-step graze_1m run (1.05M particles), free-running, DSPH_MIXED
+    against the FP64 device path (which follows the oracle to ~1e-14 here over 20
+    steps, above): the impactor aggregates the reference driver prints."""
+    a, a64, steps = whole_run_mixed_vs_fp64(ics.graze_1m(), 500)
+    assert abs(steps - 500) <= 10
+    for k in ("x", "z"):
+        assert abs(a[k] - a64[k]) <= 1e-5 * abs(a64[k]), (k, a[k], a64[k])
+    assert abs(a["speed"] - a64["speed"]) <= 1e-4 * a64["speed"]
+    assert abs(a["vz"] - a64["vz"]) <= 1e-4 * a64["speed"]
+    assert abs(a["melt"] - a64["melt"]) <= 1e-3
+    assert abs(a["umax"] - a64["umax"]) <= 0.1 * a64["umax"]
+
+
+def test_layered_4m_whole_run_mixed_follows_fp64(gpu):
+    """The whole 1000-step layered_4m run (4.0M particles), free-running: the FP64
+    device path, which follows the oracle to ~1e-14 on this workload (above),
+    sets the end time with its 1000 CFL steps; DSPH_MIXED runs to the same time
+    with its own steps.  The impactor aggregates the reference driver prints
+    agree in centroid and mean speed to ~1e-6; the melt fraction (u > 1 MJ/kg)
+    and the largest u of the impactor are extremes of single particles in a
+    violent contact and are bounded loosely (measured 1.2 % and 4 %)."""
+    a, a64, steps = whole_run_mixed_vs_fp64(ics.layered_4m(), 1000)
+    assert abs(steps - 1000) <= 20
     for k in ("x", "z"):
         assert abs(a[k] - a64[k]) <= 1e-5 * abs(a64[k]), (k, a[k], a64[k])
     assert abs(a["speed"] - a64["speed"]) <= 1e-4 * a64["speed"]
