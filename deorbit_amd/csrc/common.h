@@ -43,6 +43,9 @@ constexpr int kSliceBlock = 48;
 // record arrays: the screens read candidates in groups of four without
 // clamping the group to its run (slots past the run never count).
 constexpr int kRecordPad = 4;
+// Particles per context: record k of any record array (at most 32 bytes)
+// then sits below 2^32 bytes from the array's base (rec_at).
+constexpr int64_t kMaxParticles = ((int64_t)1 << 27) - 64;
 constexpr double kSliceRatio = 1.0 / 4.0;
 __device__ __host__ __forceinline__ int block_start(int cz) { return cz - cz % kSliceBlock; }
 // z of a block's frame origin in slices: its middle, so FP32 offsets from it
@@ -174,6 +177,15 @@ __device__ __forceinline__ unsigned long long double_to_bits(double d) {
 }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Record k of a record array, addressed by a 32-bit byte offset from the
+// array's base (the base stays in scalar registers and one 32-bit offset
+// serves every array read at the same k, instead of a 64-bit address sum per
+// load); kMaxParticles keeps k * sizeof(T) below 2^32.
+template <class T>
+__device__ __forceinline__ T rec_at(const T* base, uint32_t k) {
+    return *(const T*)((const char*)base + k * (uint32_t)sizeof(T));
+}
 
 __device__ __forceinline__ double wave_max_d(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
@@ -383,19 +395,34 @@ __device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
 // largest tile up to kColTileMax columns wide whose padding stays within a
 // tenth of the segment's smaller column count.
 constexpr int kColTileMax = 16;
-__device__ __host__ __forceinline__ uint32_t col_index(const SegGrid& g, int cx, int cy) {
+// (G: SegGrid, or WalkGrid below: any type with these fields)
+template <class G>
+__device__ __host__ __forceinline__ uint32_t col_index(const G& g, int cx, int cy) {
     const uint32_t ux = (uint32_t)cx, uy = (uint32_t)cy, sh = (uint32_t)g.tile_sh, m = (1u << sh) - 1u;
     return (((((ux >> sh) * (uint32_t)g.tiles_y + (uy >> sh)) << sh) + (ux & m)) << sh) + (uy & m);
 }
 // Column-major key: columns in tile order, slices z fastest within a column.
-__device__ __host__ __forceinline__ uint32_t cell_key(const SegGrid& g, int cx, int cy, int cz) {
+template <class G>
+__device__ __host__ __forceinline__ uint32_t cell_key(const G& g, int cx, int cy, int cz) {
     return g.cell_base + col_index(g, cx, cy) * (uint32_t)g.dim[2] + (uint32_t)cz;
 }
 // Index of the bound of h of the block holding slice cz of column (cx, cy)
 // (Work::cell_hmax).
-__device__ __host__ __forceinline__ uint32_t block_key(const SegGrid& g, int cx, int cy, int cz) {
+template <class G>
+__device__ __host__ __forceinline__ uint32_t block_key(const G& g, int cx, int cy, int cz) {
     return g.blk_base + col_index(g, cx, cy) * (uint32_t)g.nblk + (uint32_t)(cz / kSliceBlock);
 }
+// The fields of a segment's sort grid that a column walk reads, copied into
+// registers once per walk (read through a SegGrid reference inside the walk's
+// loops, they would be loaded again at every column).
+struct WalkGrid {
+    uint32_t cell_base, blk_base;
+    int32_t nblk, tiles_y, tile_sh;
+    int32_t dim[3];
+    __device__ __host__ __forceinline__ explicit WalkGrid(const SegGrid& g)
+        : cell_base(g.cell_base), blk_base(g.blk_base), nblk(g.nblk), tiles_y(g.tiles_y), tile_sh(g.tile_sh),
+          dim{g.dim[0], g.dim[1], g.dim[2]} {}
+};
 __device__ __host__ __forceinline__ uint32_t cellc_pack(int cx, int cy, int cz) {
     return (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);
 }

@@ -140,7 +140,7 @@ class Engine : public EngineBase {
         free_all();
         n_ = (int64_t)hp.mass.size();
         if (n_ <= 0) throw Err{DSPH_E_ARG, "no particles"};
-        if (n_ > (int64_t)0x7fffffff) throw Err{DSPH_E_RANGE, "more than 2^31-1 particles"};
+        if (n_ > kMaxParticles) throw Err{DSPH_E_RANGE, "more than 134,217,664 particles in one context"};
         alloc_carried(bufs_[0]);
         alloc_carried(bufs_[1]);
         cur_ = 0;
@@ -1432,7 +1432,7 @@ int dsph_upload(dsph_ctx* c, int64_t n, const double* pos3, const double* vel3, 
         const int nmat = (int)c->mats.size();
         int last_seg = hp.seg.empty() ? 0 : hp.seg.back();
         const double h_init_default = c->segp[0].h_init;
-        require((int64_t)hp.mass.size() + n <= (int64_t)0x7fffffff, DSPH_E_RANGE, "more than 2^31-1 particles");
+        require((int64_t)hp.mass.size() + n <= kMaxParticles, DSPH_E_RANGE, "more than 134,217,664 particles in one context");
         for (int64_t i = 0; i < n; i++) {           // validate everything before appending anything
             const int s = seg ? seg[i] : 0;
             require(s >= 0 && s < nseg, DSPH_E_ARG, "segment index out of range");

@@ -348,11 +348,15 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                                            R ozs, R reach_max, RF&& reach_of, F&& fn) {
     const R rmax2 = reach_max * reach_max;
     if constexpr (!PER) {
+        // grid fields in registers; the reach trouble flag is raised once after
+        // the walk (an atomic inside the loops would make every column reload them)
+        const WalkGrid G(g);
+        bool far = false;
         const R wx = (R)g.cw[0], wy = (R)g.cw[1], wz = (R)g.cw[2], iwz = (R)g.inv_cw[2];
         int rx = (int)ceil((double)reach_max * g.inv_cw[0]), ry = (int)ceil((double)reach_max * g.inv_cw[1]);
         rx = rx < 1 ? 1 : (rx > kMaxStencil ? kMaxStencil : rx);
         ry = ry < 1 ? 1 : (ry > kMaxStencil ? kMaxStencil : ry);
-        const int fi = block_start(cz), zmax = g.dim[2] - 1;
+        const int fi = block_start(cz), zmax = G.dim[2] - 1;
         // With MIR, lanes in the upper half of their column along x (y) take the
         // columns in mirrored order along x (y): at each step of the walk the
         // wave's lanes then sit in columns where they have similar numbers of
@@ -362,18 +366,18 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
         for (int ex = -rx; ex <= rx; ex++) {
             const int dx = mx * ex;
             const int nx = cx + dx;
-            if ((unsigned)nx >= (unsigned)g.dim[0]) continue;
+            if ((unsigned)nx >= (unsigned)G.dim[0]) continue;
             const R gx = cell_gap(dx, ox, wx);
             const R gx2 = gx * gx;
             if (gx2 >= rmax2) continue;
             for (int ey = -ry; ey <= ry; ey++) {
                 const int dy = my * ey;
                 const int ny = cy + dy;
-                if ((unsigned)ny >= (unsigned)g.dim[1]) continue;
+                if ((unsigned)ny >= (unsigned)G.dim[1]) continue;
                 const R gy = cell_gap(dy, oy, wy);
                 const R gxy = gx2 + gy * gy;
                 if (gxy >= rmax2) continue;
-                const uint32_t col = cell_key(g, nx, ny, 0);
+                const uint32_t col = cell_key(G, nx, ny, 0);
                 R rz = f_sqrt(rmax2 - gxy);
                 int z0 = cz + ifloor((ozs - rz) * iwz - (R)kSliceMargin);
                 int z1 = cz + ifloor((ozs + rz) * iwz + (R)kSliceMargin);
@@ -385,13 +389,13 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                     // more than two blocks: only a smoothing length far beyond 2 h_init
                     // set from outside reaches this far; report it instead of
                     // summing in the wrong frame
-                    atomicOr(&w.st[&g - w.grid].bad, 8);
+                    far = true;
                     continue;
                 }
                 R rb = reach_max;
                 if constexpr (HB) {
                     // the larger bound of h of the (at most two) blocks met
-                    const uint32_t cb = block_key(g, nx, ny, 0);
+                    const uint32_t cb = block_key(G, nx, ny, 0);
                     const R hb0 = (R)w.cell_hmax[cb + f0 / kSliceBlock];
                     const R hb1 = (f1 != f0) ? (R)w.cell_hmax[cb + f1 / kSliceBlock] : hb0;
                     rb = reach_of(hb0 > hb1 ? hb0 : hb1);
@@ -412,6 +416,7 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
                    PieceFrame{dx, dy, g0 - fi, g1 - fi});
             }
         }
+        if (far) atomicOr(&w.st[&g - w.grid].bad, 8);
         return;
     }
     AxisWalk<R, PER> X, Y, Z;
@@ -544,10 +549,9 @@ __device__ double density_walk_hp(const Carried<float>& c, const Work<float>& w,
                 const uint32_t c1 = min(c0 + 64u, k1);
                 uint64_t mask = 0;
                 for (uint32_t k = c0; k < c1; k += 4) {
-                    const float4* rec = w.dens_rec + k;
                     float4 q[4];
 #pragma unroll
-                    for (int b = 0; b < 4; b++) q[b] = rec[b];
+                    for (int b = 0; b < 4; b++) q[b] = rec_at(w.dens_rec + b, k);
                     uint32_t nib = 0;
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
@@ -557,14 +561,13 @@ __device__ double density_walk_hp(const Carried<float>& c, const Work<float>& w,
                     }
                     mask |= (uint64_t)nib << (k - c0);
                 }
-                const double4* rec = w.dens_rec64 + c0;
                 while (mask) {
                     const uint32_t ba = (uint32_t)__builtin_ctzll(mask);
                     mask &= mask - 1;
                     const bool two = mask != 0;
                     const uint32_t bb = two ? (uint32_t)__builtin_ctzll(mask) : ba;
                     mask &= mask - 1;
-                    const double4 qa = rec[ba], qb = rec[bb];
+                    const double4 qa = rec_at(w.dens_rec64, c0 + ba), qb = rec_at(w.dens_rec64, c0 + bb);
                     const double ta = term(qa), tb = term(qb);
                     s += ta;
                     s += two ? tb : 0.0;
@@ -644,10 +647,9 @@ __device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, cons
             const uint32_t c1 = min(c0 + 64u, k1);
             uint64_t mask = 0;
             for (uint32_t k = c0; k < c1; k += 4) {
-                const float4* rec = w.dens_rec + k;
                 float4 q[4];
 #pragma unroll
-                for (int b = 0; b < 4; b++) q[b] = rec[b];
+                for (int b = 0; b < 4; b++) q[b] = rec_at(w.dens_rec + b, k);
                 uint32_t nib = 0;
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
@@ -657,14 +659,13 @@ __device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, cons
                 }
                 mask |= (uint64_t)nib << (k - c0);
             }
-            const double4* rec = w.dens_rec64 + c0;
             while (mask) {
                 const uint32_t ba = (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1;
                 const bool two = mask != 0;
                 const uint32_t bb = two ? (uint32_t)__builtin_ctzll(mask) : ba;
                 mask &= mask - 1;
-                const double4 qa = rec[ba], qb = rec[bb];
+                const double4 qa = rec_at(w.dens_rec64, c0 + ba), qb = rec_at(w.dens_rec64, c0 + bb);
                 add(qa, c0 + ba, true);
                 add(qb, c0 + bb, two);
             }
@@ -679,7 +680,7 @@ __device__ double dens_band_hp(const Work<float>& w, const SegGrid& g, const Den
     double s = 0.0;
     const uint32_t bstride = blockDim.x;
     for (int b = 0; b < M.nb; b++) {
-        const double4 q = w.dens_rec64[band[b * bstride]];
+        const double4 q = rec_at(w.dens_rec64, band[b * bstride]);
         double rx = H.x - q.x, ry = H.y - q.y, rz = H.z - q.z;
         if (PER && L.per) min_image(g, rx, ry, rz);
         const double r = sqrt_seeded(rx * rx + ry * ry + rz * rz);
@@ -716,8 +717,8 @@ __device__ void density_walk_mom(const Work<double>& w, const SegGrid& g, const 
         // slots past the run read the records that follow (kRecordPad zeroed
         // records past the last particle) and add nothing
         for (uint32_t k = k0; k < k1; k += 4) {
-            const double4* rec = w.dens_rec + k;
-            const double4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
+            const double4 q0 = rec_at(w.dens_rec, k), q1 = rec_at(w.dens_rec + 1, k), q2 = rec_at(w.dens_rec + 2, k),
+                          q3 = rec_at(w.dens_rec + 3, k);
             add(q0, k, true);
             add(q1, k + 1, k + 1 < k1);
             add(q2, k + 2, k + 2 < k1);
@@ -733,7 +734,7 @@ __device__ double dens_band64(const Work<double>& w, const SegGrid& g, const Den
     double s = 0.0;
     const uint32_t bstride = blockDim.x;
     for (int b = 0; b < M.nb; b++) {
-        const double4 q = w.dens_rec[band[b * bstride]];
+        const double4 q = rec_at(w.dens_rec, band[b * bstride]);
         double rx = L.me.x - q.x, ry = L.me.y - q.y, rz = L.me.z - q.z;
         if (PER && L.per) min_image(g, rx, ry, rz);
         const double r = sqrt(rx * rx + ry * ry + rz * rz);
@@ -1385,10 +1386,9 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                         const uint32_t c1 = min(c0 + 64u, k1);
                         uint64_t mask = 0;
                         for (uint32_t k = c0; k < c1; k += kForceBatch) {
-                            const typename Vec4<R>::T* rec = w.frec_pos + k;
                             typename Vec4<R>::T q[kForceBatch];
 #pragma unroll
-                            for (int b = 0; b < kForceBatch; b++) q[b] = rec[b];
+                            for (int b = 0; b < kForceBatch; b++) q[b] = rec_at(w.frec_pos + b, k);
                             uint32_t nib = 0;
 #pragma unroll
                             for (int b = 0; b < kForceBatch; b++) {
@@ -1403,9 +1403,9 @@ __global__ __launch_bounds__(256) void k_force(Carried<R> c, Work<R> w, int rema
                             const uint32_t k = c0 + (uint32_t)__builtin_ctzll(mask);
                             mask &= mask - 1;
                             R rx, ry, rz, r2, th;
-                            F.in_range(w.frec_pos[k], bx, by, k < ks ? bz0 : bz1, rx, ry, rz, r2, th);
-                            F.body(c, w, k, rx, ry, rz, r2, th, w.frec_vel[k], w.frec_th[k], w.frec_sa[k],
-                                   w.frec_sb[k]);
+                            F.in_range(rec_at(w.frec_pos, k), bx, by, k < ks ? bz0 : bz1, rx, ry, rz, r2, th);
+                            F.body(c, w, k, rx, ry, rz, r2, th, rec_at(w.frec_vel, k), rec_at(w.frec_th, k),
+                                   rec_at(w.frec_sa, k), rec_at(w.frec_sb, k));
                         }
                     }
                 });
