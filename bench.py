@@ -92,10 +92,10 @@ def profile_leg(s, steps):
 GRAPH = True     # steps replayed from a captured hipGraph (--no-graph: stream launches)
 
 
-def gpu_leg(ic, precision, preroll, warmup, steps, run_steps, autotune=True, profile_steps=0):
+def gpu_leg(ic, precision, preroll, warmup, steps, run_steps, autotune=True, profile_steps=0, tuning=None):
     """One precision's run on the schedule of the module docstring.  Returns the
     system, the K-step time, the profile of the profiled steps and the whole-run
-    timing."""
+    timing.  `tuning` (a dict of dsph_tuning fields) replaces the autotuner."""
     name = "fp64" if precision == dsph.FP64 else "mixed"
     s = dsph.make_batch(ic, precision) if isinstance(ic, list) else dsph.make_system(ic, precision)
     s.init()
@@ -103,7 +103,10 @@ def gpu_leg(ic, precision, preroll, warmup, steps, run_steps, autotune=True, pro
     d = s.diagnostics()
     log(f"{name}: {preroll} steps before the window ({t_pre:.2f} s), t={d['t'] * 1e3:.2f} ms, "
         f"umax={d['umax']:.3g} J/kg, dmax={d['dmax']:.3f}")
-    if autotune:
+    if tuning:
+        s.set_tuning(**tuning)
+        log(f"tuning ({name}, at step {preroll}): {s.tuning()}")
+    elif autotune:
         chosen = s.autotune(3)
         log(f"autotune ({name}, at step {preroll}): {chosen}")
     t_warm = timed_run(s, warmup)
@@ -227,6 +230,9 @@ def main():
                          "(for tracing tools)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the library's default tuning instead of the autotuner's choice")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed tuning instead of the autotuner, e.g. cell_factor=1.45,block_pair=64 (profiling runs: "
+                         "counter passes then time the same configuration)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (counter calibration)")
     args = ap.parse_args()
@@ -262,9 +268,15 @@ def main():
     preroll = cfg_steps // 2 if args.preroll is None else args.preroll
     run_steps = cfg_steps if args.run_steps is None else args.run_steps
     tune = not args.no_autotune
+    fixed = None
+    if args.tuning:
+        fixed = {}
+        for kv in args.tuning.split(","):
+            k, v = kv.split("=")
+            fixed[k.strip()] = float(v) if k.strip() == "cell_factor" else int(v)
 
     s, el, prof, run = gpu_leg(ic, dsph.MIXED, preroll, args.warmup, args.steps, run_steps, tune,
-                               args.profile_steps)
+                               args.profile_steps, fixed)
     value = n * args.steps / el
     ms = 1e3 * el / args.steps
     log(f"mixed: {ms:.3f} ms/step  {value / 1e6:.1f} M particle-steps/s (steps {preroll + args.warmup + 1}-"
@@ -373,6 +385,8 @@ def main():
         "data": "synthetic (seeded lattice impact IC)",
         "config": {"workload": WORKLOAD[args.workload], "particles": n, "dt": "device CFL (cfl=0.2)",
                    "graph": GRAPH, "window": f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps}"
+                   f" of {cfg_steps} (tuning {args.tuning} from step {preroll})" if fixed else
+                   f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps}"
                    f" of {cfg_steps} (autotuned at step {preroll})" if tune else
                    f"steps {preroll + args.warmup + 1}-{preroll + args.warmup + args.steps} of {cfg_steps}",
                    "t_sim_end": diag["t"]},

@@ -157,7 +157,8 @@ void dsph_destroy(dsph_ctx* ctx);
 /* ---- state ------------------------------------------------------------ */
 /* Append n particles; replaces System::add (sph.hpp:103-111) called n times
  * with cur_mat = mat[i].  seg may be NULL (all segment 0); otherwise it must
- * be non-decreasing.  Each call appends; state is (re)allocated on device. */
+ * be non-decreasing.  Each call appends; state is (re)allocated on device.
+ * A context holds at most 134,217,664 particles (DSPH_E_RANGE beyond). */
 int dsph_upload(dsph_ctx* ctx, int64_t n, const double* pos3, const double* vel3,
                 const double* mass, const double* u, const int32_t* mat, const uint8_t* fixed,
                 const int32_t* seg);
