@@ -258,6 +258,15 @@ def test_argument_errors_are_reported(gpu):
     s.init()
     s.step(0.0)
     s.synchronize()
+    # a context holds at most 134,217,664 particles (records are addressed by 32-bit
+    # byte offsets): a larger upload is refused before any array element is read
+    import ctypes as C
+    one3, one1, onei = np.zeros(3), np.zeros(1), np.zeros(1, np.int32)
+    rc = s.lib.dsph_upload(s.h, 134_217_665 - len(ic["mass"]), dsph._ptr(one3, C.c_double),
+                           dsph._ptr(one3, C.c_double), dsph._ptr(one1, C.c_double), dsph._ptr(one1, C.c_double),
+                           dsph._ptr(onei, C.c_int32), None, None)
+    assert rc == -5, rc
+    assert s.n == len(ic["mass"])
 
 
 def test_nonfinite_particle_is_parked_and_reported(gpu):
