@@ -1423,7 +1423,11 @@ int dsph_upload(dsph_ctx* c, int64_t n, const double* pos3, const double* vel3, 
         require(c != nullptr, DSPH_E_ARG, "null context");
         require(n > 0 && pos3 && vel3 && mass && u && mat, DSPH_E_ARG, "n > 0 and pos3, vel3, mass, u, mat are required");
         HIPC(hipSetDevice(c->device));
-        if (c->committed && !c->dirty) {
+        // Appending to particles already on the device copies them back to the host
+        // staging first; a refused upload drops that copy again, so the context keeps
+        // its device state as it was.
+        const bool fetched = c->committed && !c->dirty;
+        if (fetched) {
             with_engine(c, [&](auto& e) { e.download(c->host); return 0; });
             c->committed = false;
         }
@@ -1432,18 +1436,28 @@ int dsph_upload(dsph_ctx* c, int64_t n, const double* pos3, const double* vel3, 
         const int nmat = (int)c->mats.size();
         int last_seg = hp.seg.empty() ? 0 : hp.seg.back();
         const double h_init_default = c->segp[0].h_init;
-        require((int64_t)hp.mass.size() + n <= kMaxParticles, DSPH_E_RANGE, "more than 134,217,664 particles in one context");
-        for (int64_t i = 0; i < n; i++) {           // validate everything before appending anything
-            const int s = seg ? seg[i] : 0;
-            require(s >= 0 && s < nseg, DSPH_E_ARG, "segment index out of range");
-            require(s >= last_seg, DSPH_E_ARG, "segment indices must be non-decreasing in IC order");
-            last_seg = s;
-            require(mat[i] >= 0 && mat[i] < nmat, DSPH_E_ARG, "material index out of range");
-            require(mass[i] > 0 && std::isfinite(mass[i]), DSPH_E_ARG, "masses must be positive and finite");
-            require(std::isfinite(u[i]), DSPH_E_ARG, "internal energies must be finite");
-            for (int a = 0; a < 3; a++)
-                require(std::isfinite(pos3[3 * i + a]) && std::isfinite(vel3[3 * i + a]), DSPH_E_ARG,
-                        "positions and velocities must be finite");
+        try {
+            require((int64_t)hp.mass.size() + n <= kMaxParticles, DSPH_E_RANGE,
+                    "more than 134,217,664 particles in one context");
+            for (int64_t i = 0; i < n; i++) {           // validate everything before appending anything
+                const int s = seg ? seg[i] : 0;
+                require(s >= 0 && s < nseg, DSPH_E_ARG, "segment index out of range");
+                require(s >= last_seg, DSPH_E_ARG, "segment indices must be non-decreasing in IC order");
+                last_seg = s;
+                require(mat[i] >= 0 && mat[i] < nmat, DSPH_E_ARG, "material index out of range");
+                require(mass[i] > 0 && std::isfinite(mass[i]), DSPH_E_ARG, "masses must be positive and finite");
+                require(std::isfinite(u[i]), DSPH_E_ARG, "internal energies must be finite");
+                for (int a = 0; a < 3; a++)
+                    require(std::isfinite(pos3[3 * i + a]) && std::isfinite(vel3[3 * i + a]), DSPH_E_ARG,
+                            "positions and velocities must be finite");
+            }
+        } catch (...) {
+            if (fetched) {
+                HostParticles empty;
+                std::swap(c->host, empty);
+                c->committed = true;
+            }
+            throw;
         }
         for (int64_t i = 0; i < n; i++) {
             const int s = seg ? seg[i] : 0;

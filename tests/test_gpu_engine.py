@@ -266,7 +266,15 @@ def test_argument_errors_are_reported(gpu):
                            dsph._ptr(one3, C.c_double), dsph._ptr(one1, C.c_double), dsph._ptr(one1, C.c_double),
                            dsph._ptr(onei, C.c_int32), None, None)
     assert rc == -5, rc
+    # a refused upload leaves the particles on the device as they were
     assert s.n == len(ic["mass"])
+    with pytest.raises(dsph.DsphError):
+        s.add(ic["pos"][:5], ic["vel"][:5], ic["mass"][:5], ic["u"][:5], bad[:5])
+    assert s.n == len(ic["mass"])
+    x0 = s.get("pos")
+    s.step(0.0)
+    s.synchronize()
+    assert np.all(np.isfinite(s.get("pos"))) and not np.array_equal(s.get("pos"), x0)
 
 
 def test_nonfinite_particle_is_parked_and_reported(gpu):
