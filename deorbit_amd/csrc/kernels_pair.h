@@ -340,7 +340,7 @@ constexpr float kSliceMargin = 1e-3f;
 // within reach along z (at most two slice blocks within the tuning limits,
 // hence one frame split).  A wave's
 // lanes sit in one column and walk the same column sequence with runs of
-// similar length (MIR mirrors the sequence for half of them, below), and the
+// similar length (MIR mirrors the sequence along y for half of them, below), and the
 // z range is cut to the lane's own reach rather than to whole cubic cells.
 // Periodic segments walk cell by cell.
 template <class R, bool PER, bool HB, bool MIR = false, class RF, class F>
@@ -357,12 +357,14 @@ __device__ __forceinline__ void for_pieces(const SegGrid& g, const Work<R>& w, i
         rx = rx < 1 ? 1 : (rx > kMaxStencil ? kMaxStencil : rx);
         ry = ry < 1 ? 1 : (ry > kMaxStencil ? kMaxStencil : ry);
         const int fi = block_start(cz), zmax = G.dim[2] - 1;
-        // With MIR, lanes in the upper half of their column along x (y) take the
-        // columns in mirrored order along x (y): at each step of the walk the
-        // wave's lanes then sit in columns where they have similar numbers of
-        // pairs in range (in at most four columns), which evens out the turns of
-        // a pair body that runs over the pairs in range only.
-        const int mx = (MIR && ox >= R(0.5) * wx) ? -1 : 1, my = (MIR && oy >= R(0.5) * wy) ? -1 : 1;
+        // With MIR, lanes in the upper half of their column along y take the
+        // columns in mirrored order along y: at each step of the walk the wave's
+        // lanes then sit in (at most two) columns where they have similar numbers
+        // of pairs in range, which evens out the turns of a pair body that runs
+        // over the pairs in range only.  Along y only: the two columns a step
+        // meets are then close in memory (y runs fastest inside a column tile),
+        // and mirroring along x as well spreads a wave's loads over four.
+        const int mx = 1, my = (MIR && oy >= R(0.5) * wy) ? -1 : 1;
         for (int ex = -rx; ex <= rx; ex++) {
             const int dx = mx * ex;
             const int nx = cx + dx;
@@ -615,8 +617,9 @@ __device__ __forceinline__ uint32_t* dens_band_slots() {
 constexpr size_t dens_band_lds(int block) { return (size_t)kDensBand * (size_t)block * sizeof(uint32_t); }
 
 // density_walk_hp with the moments of the walk (reach and screen widened to
-// 2h (1 + d) for the outer band); fills M and the band slots.  Lanes that
-// emulate the reference's cells do not come here.
+// 2h (1 + d) for the outer band), in the force walk's column order (mirrored
+// along y for the lanes in the upper half of their column); fills M and the
+// band slots.  Lanes that emulate the reference's cells do not come here.
 template <bool PER>
 __device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, const DensLane<float, PER>& L,
                                     const DensHP& H, DensMom& M, uint32_t* band) {
@@ -635,7 +638,7 @@ __device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, cons
         }
     };
     const float grow = (float)(1.0 + kDensBandRel);
-    for_pieces<float, PER, false>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h * grow * 1.00001f,
+    for_pieces<float, PER, false, true>(g, w, L.cx, L.cy, L.cz, L.ox, L.oy, L.ozs, L.two_h * grow * 1.00001f,
                                   [](float) { return 0.0f; },
                                   [&](uint32_t k0, uint32_t k1, uint32_t ks, float sx, float sy, float sz0, float sz1,
                                       float, const PieceFrame&) {
