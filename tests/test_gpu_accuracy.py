@@ -128,3 +128,41 @@ def test_per_field_errors_against_the_oracle(trajectories, case, precision, gpu)
           f"difference {abs(drift - drift_ref):.1e} |", flush=True)
     assert abs(drift - drift_ref) <= 1e-6
     assert abs(a["speed"] - b["speed"]) <= 1e-3 * b["speed"] and abs(a["melt"] - b["melt"]) <= 1e-3
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_mixed_pressure_per_evaluation_on_the_state_after_100_steps(trajectories, case, gpu):
+    """The per-particle pressure bar of DSPH_MIXED (1e-4 where |P| > 10 MPa) holds
+    per evaluation on the post-impact state after 100 steps.  The mixed path's own
+    state after 100 dt-lockstep steps (positions, velocities, u, h, S, D, flaws) is
+    handed to the oracle, and both evaluate the density rounds and the EOS on it
+    (compute_density_h, compute_forces: sph.hpp:185-206, 260-318).  This separates
+    the evaluation error from the drift of the trajectories, which after 100 steps
+    sets the per-particle pressure difference of the two runs (~1.5e-4, reported by
+    test_per_field_errors_against_the_oracle)."""
+    ic, dts, _, _ = trajectories[case]
+    s = dsph.make_system(ic, dsph.MIXED)
+    s.init()
+    for dt in dts[:100]:
+        s.step(dt)
+    st = s.state(["pos", "vel", "u", "hh", "S", "D", "eps_act"])
+    o = po.make_system(ic, ic["params"], ic["materials"], seed_damage=12345, dx=ic["dx"])
+    for name, field in (("pos", po.POS), ("vel", po.VEL), ("u", po.U), ("hh", po.HH), ("S", po.S),
+                        ("D", po.D), ("eps_act", po.EPS_ACT)):
+        o.set(field, st[name])
+    o.compute_density_h()
+    o.compute_forces()
+    s.compute_density_h()
+    s.compute_forces()
+    rho_g, rho_c = s.get("rho"), o.get(po.RHO)
+    p_g, p_c = s.get("P"), o.get(po.P)
+    big = np.abs(p_c) > 1e7
+    rho_rel = float(np.max(np.abs(rho_g - rho_c) / rho_c))
+    p_rel = float(np.max(np.abs(p_g[big] - p_c[big]) / np.abs(p_c[big])))
+    h_rel = float(np.max(np.abs(s.get("hh") - o.get(po.HH)) / o.get(po.HH)))
+    print(f"| {case} | mixed, evaluated on its state after 100 steps | rho rel {rho_rel:.1e} | h rel {h_rel:.1e} | "
+          f"P rel {p_rel:.1e} over {int(big.sum())} particles above 10 MPa |", flush=True)
+    assert big.sum() > 1000
+    assert rho_rel < 1e-4 and h_rel < 1e-4
+    assert p_rel < 1e-4
+    s.close()
