@@ -326,16 +326,46 @@ def whole_run_mixed_vs_fp64(ic, steps):
     d = m.diagnostics()
     assert d["t"] == t_end and d["status"] == 0
     a = impactor_aggregates(device_fields(m), ic["mat"])
-    m.close()
     print(f"{ic['meta']['name']} whole run: mixed {d['steps']} steps, fp64 {steps} steps to t={t_end:.6e} s; " +
           " ".join(f"{k} {a[k]:.6g}/{a64[k]:.6g}" for k in a), flush=True)
+    evaluate_on_own_state(ic, m)
+    m.close()
     return a, a64, d["steps"]
+
+
+def evaluate_on_own_state(ic, m):
+    """Hand the mixed path's state at the end of a run (positions, velocities, u,
+    h, S, D, flaws) to the oracle; both evaluate the density rounds and the EOS on
+    it (compute_density_h, compute_forces: sph.hpp:185-206, 260-318).  Density, h
+    and pressure per particle (pressure where |P| > 10 MPa) must agree within
+    1e-4: the per-evaluation bar of the mixed path on the run's final state,
+    apart from the drift of the trajectories."""
+    st = m.state(["pos", "vel", "u", "hh", "S", "D", "eps_act"])
+    o = oracle_for(ic)
+    for name, field in (("pos", po.POS), ("vel", po.VEL), ("u", po.U), ("hh", po.HH), ("S", po.S),
+                        ("D", po.D), ("eps_act", po.EPS_ACT)):
+        o.set(field, st[name])
+    o.compute_density_h()
+    o.compute_forces()
+    m.compute_density_h()
+    m.compute_forces()
+    rho_c, h_c, p_c = o.get(po.RHO), o.get(po.HH), o.get(po.P)
+    del o
+    big = np.abs(p_c) > P_FLOOR
+    rho_rel = float(np.max(np.abs(m.get("rho") - rho_c) / rho_c))
+    h_rel = float(np.max(np.abs(m.get("hh") - h_c) / h_c))
+    p_rel = float(np.max(np.abs(m.get("P")[big] - p_c[big]) / np.abs(p_c[big])))
+    print(f"{ic['meta']['name']} evaluated on the mixed run's final state: rho rel {rho_rel:.1e}, h rel "
+          f"{h_rel:.1e}, P rel {p_rel:.1e} over {int(big.sum())} particles above 10 MPa", flush=True)
+    assert big.sum() > 100_000
+    assert rho_rel < RHO_PARTICLE_BAR and h_rel < 1e-4 and p_rel < P_PARTICLE_BAR
 
 
 def test_graze_1m_whole_run_mixed_follows_fp64(gpu):
     """The whole 500-step graze_1m run (1.05M particles), free-running, DSPH_MIXED
     against the FP64 device path (which follows the oracle to ~1e-14 here over 20
-    steps, above): the impactor aggregates the reference driver prints."""
+    steps, above): the impactor aggregates the reference driver prints; then the
+    oracle evaluates density and pressure on the mixed run's final state."""
     a, a64, steps = whole_run_mixed_vs_fp64(ics.graze_1m(), 500)
     assert abs(steps - 500) <= 10
     for k in ("x", "z"):
@@ -353,7 +383,8 @@ def test_layered_4m_whole_run_mixed_follows_fp64(gpu):
     with its own steps.  The impactor aggregates the reference driver prints
     agree in centroid and mean speed to ~1e-6; the melt fraction (u > 1 MJ/kg)
     and the largest u of the impactor are extremes of single particles in a
-    violent contact and are bounded loosely (measured 1.2 % and 4 %)."""
+    violent contact and are bounded loosely (measured 1.2 % and 4 %).  The
+    oracle then evaluates density and pressure on the mixed run's final state."""
     a, a64, steps = whole_run_mixed_vs_fp64(ics.layered_4m(), 1000)
     assert abs(steps - 1000) <= 20
     for k in ("x", "z"):
