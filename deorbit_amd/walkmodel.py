@@ -20,6 +20,13 @@ candidates visited and the pairs in range, and per wave the turns of
   * ``flat``: each lane's pairs in range queued across all its columns and run
     one per turn: turns = the wave's largest count of pairs in range per lane;
   * ``ideal``: every lane busy: the wave's pairs in range / 64.
+
+``--mirror`` replays the mirrored column order of the kernels (``y``: lanes in
+the upper half of their column along y take the columns in mirrored order along
+y; ``xy``: along x as well; ``none``: the common order): a turn of the walk is a
+step of the stencil loop, whatever column each lane visits at that step, and
+``columns_per_step`` counts the distinct columns a wave's lanes visit at a step
+(the spread of its loads across memory).
 """
 from __future__ import annotations
 
@@ -48,7 +55,7 @@ def dump(workload: str, steps: int, out: str):
     s.close()
 
 
-def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1) -> dict:
+def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1, mirror: str = "none") -> dict:
     d = np.load(path)
     P = d["pos"].astype(np.float64)
     H = d["h"].astype(np.float64)
@@ -69,9 +76,10 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
     hmax = H.max()
     rng = np.random.default_rng(seed)
     waves = rng.choice(len(P) // 64, min(nwaves, len(P) // 64), replace=False)
-    tot = dict(cand=0, inr=0, slot=0, mask=0, flat=0, ideal=0)
+    tot = dict(cand=0, inr=0, slot=0, mask=0, flat=0, ideal=0, steps=0, colsteps=0)
     for wv in waves:
         cols = {}
+        visited = {}
         inr_wave = 0
         lane_inr = np.zeros(64, int)
         for lane, i in enumerate(range(wv * 64, wv * 64 + 64)):
@@ -80,14 +88,18 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
             cx, cy, cz = C[i]
             ox, oy, ozs = P[i, 0] - cx * w, P[i, 1] - cy * w, P[i, 2] - cz * wz
             rx = min(int(math.ceil(reach / w)), 16)
-            for dx in range(-rx, rx + 1):
+            mx = -1 if mirror == "xy" and ox >= 0.5 * w else 1
+            my = -1 if mirror in ("y", "xy") and oy >= 0.5 * w else 1
+            for ex in range(-rx, rx + 1):
+                dx = mx * ex
                 nx = cx + dx
                 if not 0 <= nx < dim[0]:
                     continue
                 gx = max(dx * w - ox, ox - (dx + 1) * w, 0.0)
                 if gx * gx >= reach * reach:
                     continue
-                for dy in range(-rx, rx + 1):
+                for ey in range(-rx, rx + 1):
+                    dy = my * ey
                     ny = cy + dy
                     if not 0 <= ny < dim[1]:
                         continue
@@ -119,7 +131,8 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
                         nin = int((r < 2 * hi).sum())
                     else:
                         nin = int(((r < hi + H[k0:k1]) & (r > 0)).sum())
-                    cols.setdefault((dx, dy), np.zeros((64, 2), int))[lane] = (k1 - k0, nin)
+                    cols.setdefault((ex, ey), np.zeros((64, 2), int))[lane] = (k1 - k0, nin)
+                    visited.setdefault((ex, ey), set()).add((nx, ny))
                     inr_wave += nin
                     lane_inr[lane] += nin
                     tot["cand"] += k1 - k0
@@ -129,15 +142,18 @@ def model(path: str, cell_factor: float, which: str, nwaves: int, seed: int = 1)
         tot["mask"] += float(A[:, :, 1].max(1).sum())
         tot["flat"] += int(lane_inr.max())
         tot["ideal"] += math.ceil(inr_wave / 64)
+        tot["steps"] += len(visited)
+        tot["colsteps"] += sum(len(v) for v in visited.values())
     n = len(waves) * 64
-    return {"particles": len(P), "waves": len(waves), "cell_factor": cell_factor, "pass": which,
+    return {"particles": len(P), "waves": len(waves), "cell_factor": cell_factor, "pass": which, "mirror": mirror,
             "h_min": float(H.min()), "h_median": float(np.median(H)), "h_max": float(hmax),
             "candidates_per_particle": tot["cand"] / n, "pairs_per_particle": tot["inr"] / n,
             "candidates_per_pair": tot["cand"] / max(1, tot["inr"]),
             "slot_turns_per_wave": tot["slot"] / len(waves), "mask_turns_per_wave": tot["mask"] / len(waves),
             "flat_turns_per_wave": tot["flat"] / len(waves),
             "ideal_turns_per_wave": tot["ideal"] / len(waves),
-            "mask_lane_use": tot["inr"] / 64 / max(1.0, tot["mask"])}
+            "mask_lane_use": tot["inr"] / 64 / max(1.0, tot["mask"]),
+            "columns_per_step": tot["colsteps"] / max(1, tot["steps"])}
 
 
 def main():
@@ -150,11 +166,12 @@ def main():
     ap.add_argument("--cell-factor", type=float, default=1.5)
     ap.add_argument("--pass", dest="which", default="force", choices=["force", "density"])
     ap.add_argument("--waves", type=int, default=150)
+    ap.add_argument("--mirror", default="none", choices=["none", "y", "xy"])
     a = ap.parse_args()
     if a.what == "dump":
         dump(a.workload, a.steps, a.out)
         return
-    r = model(a.path, a.cell_factor, a.which, a.waves)
+    r = model(a.path, a.cell_factor, a.which, a.waves, mirror=a.mirror)
     for k, v in r.items():
         print(f"{k:26s} {v:.4g}" if isinstance(v, float) else f"{k:26s} {v}")
 
