@@ -646,9 +646,14 @@ __device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, cons
         L.frame(sx, sy, sz0, bx, by, bz0);
         L.frame(sx, sy, sz1, bx, by, bz1);
         const float th2 = L.th2 * (grow * grow * 1.00002f);
-        for (uint32_t c0 = k0; c0 < k1; c0 += 64u) {
-            const uint32_t c1 = min(c0 + 64u, k1);
-            uint64_t mask = 0;
+        // Chunks of 32 candidates (32-bit masks): a run longer than a chunk costs
+        // the wave the turns of both chunks' busiest lanes, but the mask
+        // bookkeeping per pair turn is half that of 64-bit masks.  The terms are
+        // added in increasing k either way (the spare second term of a chunk's
+        // last turn adds exact zeros), so the sums do not depend on the chunking.
+        for (uint32_t c0 = k0; c0 < k1; c0 += 32u) {
+            const uint32_t c1 = min(c0 + 32u, k1);
+            uint32_t mask = 0;
             for (uint32_t k = c0; k < c1; k += 4) {
                 float4 q[4];
 #pragma unroll
@@ -660,13 +665,13 @@ __device__ void density_walk_hp_mom(const Work<float>& w, const SegGrid& g, cons
                     if (PER && L.per) min_image(g, rx, ry, rz);
                     nib |= (uint32_t)((k + b < c1) & (rx * rx + ry * ry + rz * rz < th2)) << b;
                 }
-                mask |= (uint64_t)nib << (k - c0);
+                mask |= nib << (k - c0);
             }
             while (mask) {
-                const uint32_t ba = (uint32_t)__builtin_ctzll(mask);
+                const uint32_t ba = (uint32_t)__builtin_ctz(mask);
                 mask &= mask - 1;
                 const bool two = mask != 0;
-                const uint32_t bb = two ? (uint32_t)__builtin_ctzll(mask) : ba;
+                const uint32_t bb = two ? (uint32_t)__builtin_ctz(mask) : ba;
                 mask &= mask - 1;
                 const double4 qa = rec_at(w.dens_rec64, c0 + ba), qb = rec_at(w.dens_rec64, c0 + bb);
                 add(qa, c0 + ba, true);
